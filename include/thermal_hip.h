@@ -1,0 +1,185 @@
+/* thermal_hip.h - C ABI of the MI355X (gfx950) MG-PCG thermal solver.
+ *
+ * Drop-in boundary for the GPU path of cuda-thermal: the model builder of
+ * src/stack3d.h and src/case3d.h, and the steady / backward-Euler solver
+ * concept shared by CpuPcg and GpuPcg (src/pcg3d.h:60-163), which
+ * run_transient (src/transient3d.h:48-90) consumes. Plain pointers and sizes
+ * only. Every function returns TH_OK (0) or an error code; th_last_error()
+ * returns the message of the last failure on the calling thread. No function
+ * exits the process: the CLI turns a failure into message + exit(1), as the
+ * reference's CUDA_CHECK does (src/cuda_check.h:8-16).
+ *
+ * Arrays crossing this boundary are in the reference's natural layout:
+ * cell (i, j, l) at ((l * ny + j) * nx + i); gB has nx * ny entries.
+ * Temperatures are rises above ambient (K), as in the reference.
+ */
+#ifndef THERMAL_HIP_H
+#define THERMAL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TH_API __attribute__((visibility("default")))
+
+enum {
+    TH_OK = 0,
+    TH_ERR_ARG = 1,           /* invalid argument */
+    TH_ERR_HIP = 2,           /* a HIP runtime call failed */
+    TH_ERR_NOT_FINALIZED = 3, /* model used before th_model_finalize (pcg3d_gpu.cu:347-350) */
+    TH_ERR_CASE = 4,          /* malformed case folder (case3d.h:86-155) */
+    TH_ERR_ZLIMIT = 5,        /* more than TH_MAXZ z-planes (stack3d.h:95-98) */
+    TH_ERR_NO_CELLS = 6       /* hotspot covers no cells (stack3d.h:242) */
+};
+
+#define TH_MAXZ 64 /* stack3d.h:34 */
+
+TH_API const char* th_last_error(void);
+TH_API const char* th_version(void);
+
+/* ---- model: stack3d.h:36-298, case3d.h:24-190 -------------------------- */
+typedef struct th_model th_model;
+
+/* build_model (stack3d.h:81-102): layers bottom to top. names may be NULL. */
+TH_API int th_model_create(int nlayers, const char* const* names, const double* thickness_m,
+                           const double* k, const double* rho_c, const int* planes,
+                           const int* active, int nx, int ny, double Lx_m, double Ly_m,
+                           double h_top, double h_bot, th_model** out);
+/* default_model (stack3d.h:153-168): the two-tier stack, 10 mm x 10 mm. */
+TH_API int th_model_create_default(int nx, int ny, th_model** out);
+/* load_case + build_case_model (case3d.h:86-166). On TH_ERR_CASE the
+ * message names the file and the problem, like the reference's `err`. */
+TH_API int th_model_load_case(const char* dir, int nx, int ny, int with_tsv, th_model** out);
+/* example_case (case3d.h:170-190) built on nx x ny and finalized. */
+TH_API int th_model_example_case(int nx, int ny, int with_tsv, th_model** out);
+TH_API void th_model_destroy(th_model* m);
+
+TH_API int th_model_add_power_block(th_model* m, int layer, double x0, double y0, double w,
+                                    double h, double watts);          /* stack3d.h:196 */
+TH_API int th_model_add_uniform_power(th_model* m, int layer, double watts); /* :223 */
+TH_API int th_model_add_hotspot(th_model* m, int layer, double xc, double yc, double size,
+                                double watts);                         /* stack3d.h:230 */
+TH_API int th_model_add_tsv_region(th_model* m, int layer, double x0, double y0, double w,
+                                   double h, double fraction);         /* stack3d.h:210 */
+TH_API int th_model_finalize(th_model* m);                             /* stack3d.h:112 */
+TH_API int th_model_set_power(th_model* m, const double* P);           /* replace P (W/cell) */
+
+TH_API int th_model_dims(const th_model* m, int* nx, int* ny, int* nz, double* dx, double* dy);
+TH_API int th_model_layer_count(const th_model* m);
+TH_API int th_model_plane_layer(const th_model* m, int* plane_layer /* nz */);
+TH_API int th_model_layer_info(const th_model* m, int layer, double* thickness_m, double* k,
+                               double* rho_c, int* planes, int* active, const char** name);
+TH_API int th_model_settings(const th_model* m, double* Lx, double* Ly, double* h_top,
+                             double* h_bot, double* T_amb);
+/* which: "gE" "gN" "gU" "cap" (float, n), "gB" (float, nx*ny). */
+TH_API int th_model_get_f32(const th_model* m, const char* which, float* out);
+/* which: "P" "tsv_frac" (double, n), "dz" (double, nz). */
+TH_API int th_model_get_f64(const th_model* m, const char* which, double* out);
+TH_API double th_model_total_power(const th_model* m);                /* stack3d.h:246 */
+TH_API int th_model_analytic_1d(const th_model* m, double* T /* nz */); /* stack3d.h:279 */
+TH_API double th_model_heat_out(const th_model* m, const double* T);  /* stack3d.h:254 */
+
+/* What the solver reads from a finalized Model3D (pcg3d_gpu.cu:372-391).
+ * All arrays are copied by th_solver_create. */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy;           /* lateral cell size (m), for the coarse operators */
+    const float* gE;         /* n: conductance to +x (0 on the last column) */
+    const float* gN;         /* n: to +y (0 on the last row) */
+    const float* gU;         /* n: to +z, or to ambient on the top plane */
+    const float* gB;         /* nx * ny: bottom plane to ambient */
+    const float* cap;        /* n: heat capacity (J/K) */
+    const double* P;         /* n: power (W) */
+} th_model_desc;
+
+/* Descriptor pointing into a finalized model (valid while the model lives). */
+TH_API int th_model_describe(const th_model* m, th_model_desc* out);
+
+/* ---- solver: pcg3d.h:60-163 ---------------------------------------------- */
+typedef struct th_solver th_solver;
+
+enum { TH_PRECOND_JACOBI = 0, TH_PRECOND_MG = 1 };           /* pcg3d.h:54 */
+enum {
+    TH_PREC_MIXED = 0, /* FP32 conductances, factors and V-cycle; FP64 PCG */
+    TH_PREC_FP64 = 1   /* FP64 everywhere (conductances widened from float) */
+};
+enum { TH_BETA_FLETCHER_REEVES = 0, TH_BETA_FLEXIBLE = 1 };
+
+typedef struct {
+    int precond;    /* TH_PRECOND_* */
+    int precision;  /* TH_PREC_* */
+    int beta;       /* TH_BETA_*; the reference uses Fletcher-Reeves (pcg3d_cpu.cpp:179) */
+    int use_graphs; /* replay iterations from hipGraphs (gpu_use_graphs, pcg3d.h:167) */
+    int device;     /* HIP device ordinal */
+} th_solver_opts;
+
+TH_API void th_solver_default_opts(th_solver_opts* o);
+
+typedef struct {
+    int iters;
+    double relres;  /* ||r|| / ||b|| of the PCG recurrence */
+    int converged;
+} th_stats;          /* PcgStats, pcg3d.h:45-49 */
+
+TH_API int th_solver_create(const th_model_desc* m, const th_solver_opts* o, th_solver** out);
+TH_API void th_solver_destroy(th_solver* s); /* never fails loudly (pcg3d_gpu.cu:495-509) */
+TH_API int th_solver_set_dt(th_solver* s, double dt);       /* 0 = steady */
+TH_API int th_solver_set_x_zero(th_solver* s);
+TH_API int th_solver_set_x(th_solver* s, const double* x);  /* warm start / initial guess */
+TH_API int th_solver_set_rhs_steady(th_solver* s);          /* b = P */
+TH_API int th_solver_set_rhs_transient(th_solver* s);       /* b = P + C/dt x */
+TH_API int th_solver_solve(th_solver* s, double tol, int maxit, th_stats* st);
+TH_API int th_solver_get_x(th_solver* s, double* x);
+TH_API int th_solver_set_power(th_solver* s, const double* P);
+/* Device pointer to x in the natural layout (refreshed by this call; valid
+ * until the next solver call). */
+TH_API int th_solver_device_x(th_solver* s, const double** d_x);
+/* ||P - A T|| / ||P|| of the steady operator at the current x, in FP64 on
+ * the device (pcg3d_cpu.cpp:207-220). */
+TH_API int th_solver_true_relres(th_solver* s, double* out);
+/* heat_out (stack3d.h:254) at the current x, on the device. */
+TH_API int th_solver_heat_out(th_solver* s, double* out);
+TH_API int th_solver_nlevels(th_solver* s);
+/* z = M^-1 r: one preconditioner application (the multigrid V-cycle in the
+ * solver's precision, or Jacobi) for the current dt, host arrays in the
+ * natural layout (CpuPcg::precondition, pcg3d_cpu.cpp:115-124). */
+TH_API int th_solver_precondition(th_solver* s, const double* r, double* z);
+/* y = (A + C/dt) x in FP64 for the current dt (operator3d.h:109-112). */
+TH_API int th_solver_apply_operator(th_solver* s, const double* x, double* y);
+
+/* Steady solve from T = 0 with timing (gpu_pcg, pcg3d_gpu.cu:686-719). */
+typedef struct {
+    int iters;
+    double relres;
+    int converged;
+    double setup_ms, solve_ms, total_ms;
+} th_solve_result;
+TH_API int th_steady_solve(const th_model_desc* m, const th_solver_opts* o, double tol, int maxit,
+                           double* T, th_solve_result* res);
+
+/* ---- transient: transient3d.h:19-90 -------------------------------------- */
+typedef struct {
+    double dt0, growth, dt_max, t_end; /* TransientOptions (transient3d.h:19-27) */
+    double tol;
+    int maxit;
+    int max_steps;                     /* > 0: also stop after this many steps */
+} th_transient_opts;
+TH_API void th_transient_default_opts(th_transient_opts* o);
+
+typedef struct {
+    double t, dt;
+    int iters;
+    double T_max, q_out, q_stored;
+} th_step_record; /* StepRecord, transient3d.h:29-35 */
+
+/* run_transient: starts at ambient; one backward-Euler step per record.
+ * Energy terms are computed on the device in FP64. T_final (optional, n)
+ * receives the last field. */
+TH_API int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* recs,
+                            int rec_cap, int* nsteps, double* T_final, int* all_converged,
+                            double* solve_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THERMAL_HIP_H */

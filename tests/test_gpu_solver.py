@@ -1,0 +1,183 @@
+"""GPU parity tests: the HIP solver through the C ABI against the oracle (the
+CPU restatement of the reference's pcg3d_cpu path) on the same inputs.
+
+Bars (the reference's own checks, main3d.cu:94-282, tightened to the
+north-star numbers): solution within 5e-4 C max-abs of the FP64 oracle;
+true relative residual <= 1e-10; energy balance to 1e-6; the exact 1D
+resistor chain under uniform power; FP64 operator apply and the FP32 / FP64
+V-cycle bit-identical to the oracle's (same operation order, no FMA
+contraction on either side).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import thermal_amd as T
+
+pytestmark = pytest.mark.gpu
+
+TOL_C = 5e-4  # max-abs difference vs the FP64 oracle (C)
+
+
+def hotspot(n, ny=None, at=(3e-3, 3e-3)):
+    m = T.Model.default(n, ny)
+    m.add_uniform_power(0, 22.0)
+    m.add_hotspot(0, at[0], at[1], 1e-3, 3.0)
+    m.add_uniform_power(2, 25.0)
+    return m.finalize()
+
+
+@pytest.fixture(scope="module")
+def example_odd():
+    """The reference's example case (TSVs, off-grid blocks) on an odd grid."""
+    return T.Model.example_case(37, 23)
+
+
+def rng_vec(n, seed=0):
+    return np.random.default_rng(seed).standard_normal(n)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("dt", [0.0, 1e-4])
+def test_operator_bitwise(example_odd, precision, dt):
+    a = example_odd.arrays()
+    s = T.Solver(example_odd, precision=precision)
+    s.set_dt(dt)
+    o = O.Solver(a, "mg")
+    o.set_dt(dt)
+    x = rng_vec(example_odd.n, 1)
+    assert np.array_equal(s.apply(x), o.apply(x))
+
+
+@pytest.mark.parametrize("precision,mg_double", [("mixed", False), ("fp64", True)])
+@pytest.mark.parametrize("dt", [0.0, 1e-3])
+def test_vcycle_bitwise(example_odd, precision, mg_double, dt):
+    a = example_odd.arrays()
+    s = T.Solver(example_odd, precision=precision)
+    s.set_dt(dt)
+    o = O.Solver(a, "mg", mg_double=mg_double)
+    o.set_dt(dt)
+    r = rng_vec(example_odd.n, 2)
+    zg, zo = s.precondition(r), o.precond(r)
+    assert np.array_equal(zg, zo), np.abs(zg - zo).max()
+
+
+def test_jacobi_precond_matches(example_odd):
+    a = example_odd.arrays()
+    s = T.Solver(example_odd, precond="jacobi")
+    o = O.Solver(a, "jacobi")
+    r = rng_vec(example_odd.n, 3)
+    assert np.array_equal(s.precondition(r), o.precond(r))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("beta", ["flexible", "fr"])
+def test_steady_vs_oracle(precision, beta):
+    m = hotspot(64)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-10, mg_double=(precision == "fp64"), flexible=(beta == "flexible"))
+    s = T.Solver(m, precision=precision, beta=beta)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 200)
+    x = s.get_x()
+    assert st["converged"]
+    assert abs(st["iters"] - ref["iters"]) <= 1
+    assert np.abs(x - ref["T"]).max() < TOL_C
+    assert np.abs(x - ref["T"]).max() < 1e-8 * ref["T"].max()
+    tr = s.true_relres()
+    assert tr <= 1.5e-10
+    assert abs(tr - O.true_relres(a, x)) <= 1e-3 * tr + 1e-15
+
+
+def test_steady_odd_grid_with_tsvs(example_odd):
+    a = example_odd.arrays()
+    ref = O.steady(a, tol=1e-10, flexible=True)
+    r = T.steady_solve(example_odd, tol=1e-10)
+    assert r["converged"]
+    assert np.abs(r["T"] - ref["T"]).max() < TOL_C
+    assert O.true_relres(a, r["T"]) <= 1.5e-10
+    P = example_odd.total_power()
+    assert abs(O.heat_out(a, r["T"]) - P) / P < 1e-6
+
+
+def test_uniform_power_is_1d_chain():
+    m = T.Model.default(48)
+    m.add_uniform_power(0, 25.0)
+    m.add_uniform_power(2, 25.0)
+    m.finalize()
+    r = T.steady_solve(m, tol=1e-10)
+    Tz = m.analytic_1d()
+    err = np.abs(r["T"].reshape(m.nz, -1) - Tz[:, None]).max()
+    assert err / Tz.max() < 1e-5
+    P = m.total_power()
+    assert abs(m.heat_out(r["T"]) - P) / P < 1e-6
+
+
+def test_jacobi_steady_vs_oracle():
+    m = hotspot(24)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-9, precond="jacobi")
+    r = T.steady_solve(m, tol=1e-9, precond="jacobi")
+    assert r["converged"]
+    assert abs(r["iters"] - ref["iters"]) <= 2
+    assert np.abs(r["T"] - ref["T"]).max() < 1e-6 * ref["T"].max()
+
+
+def test_zero_power_and_maxit():
+    m = T.Model.default(16).finalize()
+    s = T.Solver(m)
+    s.set_x(np.ones(m.n))
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 100)
+    assert st["converged"] and st["iters"] == 0
+    assert not s.get_x().any()
+    m2 = hotspot(32)
+    s2 = T.Solver(m2)
+    s2.set_x_zero()
+    s2.set_rhs_steady()
+    st2 = s2.solve(1e-12, 2)
+    assert not st2["converged"] and st2["iters"] == 2
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_transient_vs_oracle(graphs):
+    m = hotspot(32)
+    a = m.arrays()
+    ref = O.run_transient(a, t_end=0.02, growth=1.2, tol=1e-9, flexible=True)
+    s = T.Solver(m, graphs=graphs)
+    g = T.run_transient(s, t_end=0.02, growth=1.2, tol=1e-9)
+    assert g["converged"] and len(g["t"]) == len(ref["t"])
+    assert np.abs(g["iters"] - ref["iters"]).max() <= 1
+    assert np.abs(g["T"] - ref["T"]).max() < 1e-7 * ref["T"].max()
+    assert np.allclose(g["T_max"], ref["T_max"], rtol=1e-7)
+    P = m.total_power()
+    eb = np.abs(g["q_out"] + g["q_stored"] - P) / P
+    assert eb.max() < 1e-6
+
+
+def test_graphs_match_direct_launches():
+    """Replayed graphs run the same kernels with deterministic reductions, so
+    a transient with 1/dt changing between replays is bit-identical."""
+    m = hotspot(40)
+    out = []
+    for graphs in (True, False):
+        s = T.Solver(m, graphs=graphs)
+        out.append(T.run_transient(s, t_end=0.01, growth=1.3, tol=1e-9))
+    assert np.array_equal(out[0]["T"], out[1]["T"])
+    assert np.array_equal(out[0]["iters"], out[1]["iters"])
+
+
+def test_set_power_and_warm_start():
+    m = hotspot(32)
+    s = T.Solver(m)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    s.solve(1e-10)
+    x1 = s.get_x()
+    s.set_power(2.0 * m.get("P"))
+    s.set_rhs_steady()
+    st = s.solve(1e-10)  # warm start from x1
+    x2 = s.get_x()
+    assert st["converged"]
+    assert np.abs(x2 - 2.0 * x1).max() < 1e-8 * x2.max()
