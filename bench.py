@@ -1,0 +1,199 @@
+"""Benchmark: ms per 1024^2 x 16 steady solve to true relative residual
+1e-10, plain FP64 MG-PCG vs the mixed-precision solver, on one MI355X.
+
+A step is one complete steady solve of the TSV stack (1024 x 1024 x 16
+cells, seeded random floorplan, TSV clusters) from T = 0, with the model
+already resident in HBM. Prints one JSON line (see the `line` dict below);
+progress goes to stderr.
+
+  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--n 1024] [--no-cpu]
+  python bench.py --kernels-only   # sweep + copy kernels only, for rocprofv3 --pmc
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "cuda-thermal_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import thermal_amd as T  # noqa: E402
+from thermal_amd import workloads as W  # noqa: E402
+
+METRIC = "ms per 1024²×16 solve to rel-res 1e-10: plain FP64 vs mixed+learned; %HBM"
+PEAK_GBS = 8000.0
+TOL = 1e-10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(n, seed):
+    spec = W.config2_spec(seed)
+    folder = W.write_case(spec, os.path.join(tempfile.gettempdir(), f"thermal_bench_case_{seed}"))
+    return folder, T.Model.load_case(folder, n)
+
+
+def time_solves(solver, steps, warmup):
+    for _ in range(warmup):
+        solver.set_x_zero()
+        solver.set_rhs_steady()
+        solver.solve(TOL, 500)
+    ms, st = [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        solver.set_x_zero()
+        solver.set_rhs_steady()
+        st = solver.solve(TOL, 500)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return ms, st
+
+
+def run_mode(model, name, steps, warmup, **opts):
+    t0 = time.perf_counter()
+    s = T.Solver(model, true_residual=True, **opts)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    ms, st = time_solves(s, steps, warmup)
+    x = s.get_x()
+    P = model.total_power()
+    res = dict(ms=float(np.mean(ms)), ms_min=float(np.min(ms)), iters=st["iters"], converged=st["converged"],
+               true_relres=s.true_relres(), energy_balance=abs(s.heat_out() - P) / P, setup_ms=setup_ms,
+               T_max_C=float(x.max()) + 25.0)
+    log(f"  {name:6s}: {res['ms']:.2f} ms/solve (min {res['ms_min']:.2f}), {res['iters']} iterations, "
+        f"true rel-res {res['true_relres']:.2e}, energy balance {res['energy_balance']:.1e}, setup {setup_ms:.0f} ms")
+    return s, x, res
+
+
+def cpu_baseline(folder, n, iters):
+    """The reference's CPU MG-PCG (oracle/_ref, built from the reference
+    sources) or else the oracle's restatement, single-threaded, on the same
+    case folder: a bounded sample of 2 PCG iterations, scaled to a solve of
+    `iters` iterations (plus the initial residual and V-cycle)."""
+    import oracle as O
+
+    sample_it = 2
+    t0 = time.perf_counter()
+    if O.ref_available():
+        kind = "reference"
+        m = O.RefModel.case(folder, n)
+        r = m.cpu_pcg(TOL, maxit=sample_it, precond="mg")
+        solve_ms = r["solve_ms"]
+    else:
+        kind = "port"
+        import thermal_amd as TT
+
+        a = TT.Model.load_case(folder, n).arrays()
+        s = O.Solver(a, "mg")
+        s.set_x_zero()
+        s.set_rhs_steady()
+        t1 = time.perf_counter()
+        s.solve(TOL, sample_it)
+        solve_ms = (time.perf_counter() - t1) * 1e3
+    wall = time.perf_counter() - t0
+    per_it = solve_ms / (sample_it + 1)
+    value = per_it * (iters + 1)
+    return dict(value=round(value, 1), unit="ms per solve (estimated)", cores=1, kind=kind,
+                sample=(f"{n}x{n}x16 config, {sample_it} MG-PCG iterations ({solve_ms:.0f} ms, {wall:.1f} s wall "
+                        f"incl. model and level setup) on 1 core, scaled to {iters} iterations + start"),
+                ms_per_iteration=round(per_it, 1))
+
+
+def kernels_only(n):
+    """Sweep and copy kernels only (for counter runs under rocprofv3 --pmc)."""
+    folder, m = make_case(n, 1000)
+    s = T.Solver(m, precision="mixed")
+    s.set_x_zero()
+    s.set_rhs_steady()
+    s.solve(TOL, 500)
+    for which in ("sweep", "copy"):
+        ms, nb = s.time_kernel(which, 5)
+        log(f"{which}: {ms * 1e3:.1f} us, {nb / 1e6:.1f} MB algorithmic, {nb / ms / 1e6:.0f} GB/s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single solve "
+              "does not shard across GPUs here, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.kernels_only:
+        kernels_only(args.n)
+        return
+
+    n = args.n
+    log(f"bench: {n}x{n}x16 TSV stack, seed {args.seed}, steady solve to true rel-res {TOL:g}")
+    folder, model = make_case(n, args.seed)
+    log(f"  model: {model.n} cells, {model.total_power():.2f} W")
+
+    s64, x64, r64 = run_mode(model, "fp64", args.steps, args.warmup, precision="fp64", beta="fr")
+    s64.close()
+    smx, xmx, rmx = run_mode(model, "mixed", args.steps, args.warmup, precision="mixed", beta="flexible")
+    err_mixed = float(np.abs(xmx - x64).max())
+    log(f"  max |mixed - fp64| = {err_mixed:.2e} K")
+
+    # Dominant kernel: the fine-level colour sweep of the FP32 V-cycle.
+    sweep_ms, sweep_bytes = smx.time_kernel("sweep", 50)
+    copy_ms, copy_bytes = smx.time_kernel("copy", 50)
+    apply_ms, apply_bytes = smx.time_kernel("apply", 50)
+    vc_ms, _ = smx.time_kernel("vcycle", 10)
+    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    log(f"  fine sweep: {sweep_ms * 1e3:.1f} us, {sweep_bytes / 1e6:.1f} MB -> {achieved:.0f} GB/s "
+        f"({100 * achieved / PEAK_GBS:.1f}% of 8 TB/s); copy {copy_bytes / copy_ms / 1e6:.0f} GB/s; "
+        f"apply {apply_bytes / apply_ms / 1e6:.0f} GB/s; V-cycle {vc_ms * 1e3:.0f} us")
+    traffic = None
+    cnt = os.path.join(ROOT, "profiles", "sweep_counters.json")
+    if os.path.exists(cnt):
+        with open(cnt) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("  cpu baseline (bounded sample) ...")
+        cpu = cpu_baseline(folder, n, r64["iters"])
+        log(f"  cpu: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(rmx["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 PCG + f32 V-cycle",
+        "data": "synthetic (seeded floorplan and TSV clusters)",
+        "config": {"workload": f"{n}x{n}x16 TSV stack, steady MG-PCG from T=0 to true rel-res 1e-10",
+                   "seed": args.seed, "cells": model.n, "mode": "mixed (FP32 V-cycle, FP64 flexible PCG)"},
+        "modes": {"fp64": r64, "mixed": rmx},
+        "speedup_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+        "max_abs_err_mixed_vs_fp64_K": err_mixed,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_sweep<float> fine level, one colour",
+                     "bytes_per_launch": sweep_bytes, "avg_launch_us": round(sweep_ms * 1e3, 2)},
+        "kernels_GBps": {"copy": round(copy_bytes / copy_ms / 1e6, 1), "apply": round(apply_bytes / apply_ms / 1e6, 1),
+                         "sweep": round(achieved, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,4 +1,5 @@
-// kernels.h - gfx950 kernels of the MG-PCG solver.
+// kernels.h - device-side building blocks shared by the multigrid and PCG
+// kernels of the gfx950 MG-PCG solver.
 //
 // Device layout (every vector and every multigrid level): colour-split,
 // plane-major. Within z-plane l, row j stores its red columns ((i+j) even)
@@ -10,16 +11,13 @@
 // (the colour-split layout of src/operator3d.h:15-23). Padding slots (odd nx)
 // hold zeros in every array and are never written by the sweeps.
 //
-// One thread per column: it walks the nz planes, so vertical neighbours stay
-// in registers and each plane's loads are issued back to back. The column
-// count per launch (e.g. 2^20 at 1024^2) fills the chip's 256 CUs many
-// times over.
-//
-// Reductions are deterministic: wave64 butterfly, then the waves of a block
-// through LDS, then one partial per block; the last block to finish (an
-// atomic ticket) adds the partials in block order and runs the scalar
-// epilogue (alpha, beta, ||r||, convergence), so PCG scalars never leave the
-// device and no floating-point atomics are used.
+// Reductions are deterministic and fence-free: a wave64 butterfly, the waves
+// of a block through LDS, one partial per block in a fixed slot; a separate
+// one-block kernel then adds the partials in block order and runs the scalar
+// epilogue (alpha, beta, ||r||, convergence). On gfx950 a device-scope fence
+// in every block (the "last block" pattern) writes back the XCD's L2 each
+// time and cost the fused kernels 3-5x of their bandwidth; the kernel
+// boundary gives the same ordering once per launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,7 +28,8 @@ namespace th {
 constexpr int kColsX = 64;  // threads per block along a half-row (one wave)
 constexpr int kColsY = 4;   // rows per block
 constexpr int kVecThreads = 256;
-constexpr int kMaxVecBlocks = 2048;
+constexpr int kMaxVecBlocks = 1024;
+constexpr int kFinThreads = 256;
 
 // Split-layout geometry of one level.
 struct Geo {
@@ -51,17 +50,15 @@ inline Geo make_geo(int nx, int ny, int nz) {
 }
 
 // Device-resident PCG scalars and flags (the role of the reference's sc[]
-// slots, pcg3d_gpu.cu:80-131), plus the tickets of the last-block reductions.
+// slots, pcg3d_gpu.cu:80-131).
 struct State {
-    double inv_dt;       // 1/dt of the current operator (0 = steady)
+    double inv_dt;  // 1/dt of the current operator (0 = steady)
     double tol;
     double bb, rr, rz, rzo, pAp, alpha, beta, relres;
-    double chk_rr, chk_bb;          // true-residual check
-    double e_heat, e_out, e_tmax;   // energy: sum C T, heat out, max T
+    double chk_rr, chk_bb;         // true-residual check
+    double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
     int maxit, iters, done, converged, zero_x, flexible;
-    unsigned ticket[8];
 };
-enum { TK_RES = 0, TK_APPLY, TK_UPDATE, TK_DOT, TK_CHECK, TK_ENERGY };
 
 // Conductance arrays of one level (split layout).
 template <class T>
@@ -82,17 +79,14 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Reduces NV values over the grid. Every thread of every block must call it.
-// Returns true in the one block that finished last; there, thread 0 holds the
-// grid totals in tot[]. KMAX marks which entries reduce by max instead of sum.
+// Block totals of NV values into part[block * NV + k] (bit k of KMAX set:
+// max instead of sum). Every thread of the block must call it.
 template <int NV, unsigned KMAX = 0>
-__device__ bool grid_reduce(double (&v)[NV], double* part, unsigned* ticket, double (&tot)[NV]) {
+__device__ void block_partials(double (&v)[NV], double* __restrict__ part) {
     __shared__ double sh[NV][16];
-    __shared__ int last;
     const int nthr = blockDim.x * blockDim.y;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = tid & 63, w = tid >> 6, nw = (nthr + 63) >> 6;
-    const int nblk = gridDim.x * gridDim.y * gridDim.z;
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((KMAX >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
@@ -100,60 +94,52 @@ __device__ bool grid_reduce(double (&v)[NV], double* part, unsigned* ticket, dou
 #pragma unroll
         for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
     __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = sh[k][0];
-            for (int q = 1; q < nw; ++q) s = ((KMAX >> k) & 1) ? fmax(s, sh[k][q]) : s + sh[k][q];
-            part[(size_t)bid * NV + k] = s;
-        }
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        last = (t == (unsigned)nblk - 1);
+    if (tid < NV) {
+        double s = sh[tid][0];
+        for (int q = 1; q < nw; ++q) s = ((KMAX >> tid) & 1) ? fmax(s, sh[tid][q]) : s + sh[tid][q];
+        part[(size_t)bid * NV + tid] = s;
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// Grid totals from nblk block partials, in block order (one block of
+// kFinThreads threads). Result valid in thread 0.
+template <int NV, unsigned KMAX = 0>
+__device__ void sum_partials(const double* __restrict__ part, int nblk, double (&tot)[NV]) {
+    __shared__ double sh[NV][kFinThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     double acc[NV];
-    const volatile double* vp = part;
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = ((KMAX >> k) & 1) ? -1.0e300 : 0.0;
-    for (int b = tid; b < nblk; b += nthr)
+    for (int b = tid; b < nblk; b += kFinThreads)
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            const double x = vp[(size_t)b * NV + k];
+            const double x = part[(size_t)b * NV + k];
             acc[k] = ((KMAX >> k) & 1) ? fmax(acc[k], x) : acc[k] + x;
         }
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = ((KMAX >> k) & 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
-    __syncthreads();
     if (lane == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) sh[k][w] = acc[k];
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = sh[k][0];
-            for (int q = 1; q < nw; ++q) s = ((KMAX >> k) & 1) ? fmax(s, sh[k][q]) : s + sh[k][q];
+            for (int q = 1; q < kFinThreads / 64; ++q) s = ((KMAX >> k) & 1) ? fmax(s, sh[k][q]) : s + sh[k][q];
             tot[k] = s;
         }
-        *ticket = 0u;
-    }
-    return true;
 }
 
 // ---- column addressing ------------------------------------------------------
 
 // A column (i, j) and the in-plane slots of its 4 lateral neighbours.
 struct Col {
-    int i, j, c0, pw, pe, ps, pn;
+    int c0, pw, pe, ps, pn;
     bool hw, he, hs, hn;
 };
 __device__ __forceinline__ Col make_col(const Geo& g, int i, int j) {
     Col c;
-    c.i = i;
-    c.j = j;
     const int colour = (i + j) & 1, oc = colour ^ 1, t = i >> 1;
     c.c0 = (2 * j + colour) * g.hx + t;
     c.hw = i > 0;
@@ -164,36 +150,87 @@ __device__ __forceinline__ Col make_col(const Geo& g, int i, int j) {
     c.pe = (2 * j + oc) * g.hx + ((i + 1) >> 1);
     c.ps = (2 * j - 2 + oc) * g.hx + t;
     c.pn = (2 * j + 2 + oc) * g.hx + t;
+    // Absent neighbours point at the cell itself, so loads stay in bounds.
+    if (!c.hw) c.pw = c.c0;
+    if (!c.he) c.pe = c.c0;
+    if (!c.hs) c.ps = c.c0;
+    if (!c.hn) c.pn = c.c0;
     return c;
 }
 
-// (A + C/dt) applied at cell (column c, plane l), in precision R, with the
+// Column of the row slot q (both colours) of row j; false for padding.
+__device__ __forceinline__ bool slot_column(const Geo& g, int q, int j, int& i) {
+    const int colour = q >= g.hx ? 1 : 0, t = q - colour * g.hx;
+    i = 2 * t + ((j + colour) & 1);
+    return i < g.nx;
+}
+
+// (A + C/dt) x at cell (column c, plane l) in precision R, in the
 // reference's operation order (operator3d.h:81-112): diagonal = lateral faces
 // (W, E, S, N) + up + (down or bottom ambient) + C/dt; off-diagonal = lateral
-// G x (W, E, S, N) + below + above.
+// G x (W, E, S, N) + below + above. All loads are issued before the
+// arithmetic (absent neighbours read the cell itself and are masked out).
 template <class R, class G, class V>
-__device__ __forceinline__ R apply_cell(const Geo& g, const Ops<G>& o, const V* __restrict__ x, const Col& c,
-                                        int l, R inv_dt) {
+__device__ __forceinline__ R apply_cell(const Geo& g, const G* __restrict__ gE, const G* __restrict__ gN,
+                                        const G* __restrict__ gU, const G* __restrict__ gB,
+                                        const G* __restrict__ cap, const V* __restrict__ x, const Col& c, int l,
+                                        R inv_dt) {
     const long base = (long)l * g.plane;
     const long id = base + c.c0;
+    const long idd = (l > 0) ? id - g.plane : id;
+    const long idu = (l < g.nz - 1) ? id + g.plane : id;
+    const R Gw = (R)gE[base + c.pw], Ge = (R)gE[id], Gs = (R)gN[base + c.ps], Gn = (R)gN[id];
+    const R Gu = (R)gU[id], Gd = (l > 0) ? (R)gU[idd] : (R)gB[c.c0];
+    const R xw = (R)x[base + c.pw], xe = (R)x[base + c.pe], xs = (R)x[base + c.ps], xn = (R)x[base + c.pn];
+    const R xc = (R)x[id], xd = (R)x[idd], xu = (R)x[idu];
     R d = 0, s = 0;
-    if (c.hw) {
-        const R gw = (R)o.gE[base + c.pw];
-        d += gw;
-    }
-    if (c.he) d += (R)o.gE[id];
-    if (c.hs) d += (R)o.gN[base + c.ps];
-    if (c.hn) d += (R)o.gN[id];
-    d += (R)o.gU[id];
-    d += (l > 0) ? (R)o.gU[id - g.plane] : (R)o.gB[c.c0];
-    if (inv_dt != R(0)) d = d + (R)o.cap[id] * inv_dt;
-    if (c.hw) s += (R)o.gE[base + c.pw] * (R)x[base + c.pw];
-    if (c.he) s += (R)o.gE[id] * (R)x[base + c.pe];
-    if (c.hs) s += (R)o.gN[base + c.ps] * (R)x[base + c.ps];
-    if (c.hn) s += (R)o.gN[id] * (R)x[base + c.pn];
-    if (l > 0) s += (R)o.gU[id - g.plane] * (R)x[id - g.plane];
-    if (l < g.nz - 1) s += (R)o.gU[id] * (R)x[id + g.plane];
-    return d * (R)x[id] - s;
+    if (c.hw) d += Gw;
+    if (c.he) d += Ge;
+    if (c.hs) d += Gs;
+    if (c.hn) d += Gn;
+    d += Gu;
+    d += Gd;
+    if (inv_dt != R(0)) d = d + (R)cap[id] * inv_dt;
+    if (c.hw) s += Gw * xw;
+    if (c.he) s += Ge * xe;
+    if (c.hs) s += Gs * xs;
+    if (c.hn) s += Gn * xn;
+    if (l > 0) s += Gd * xd;
+    if (l < g.nz - 1) s += Gu * xu;
+    return d * xc - s;
+}
+
+// (A + C/dt) x at cell (column c, plane l) in FP64, in flux form:
+//   sum over faces G (x_c - x_nb) + ambient links x_c + C/dt x_c.
+// Mathematically the same operator as apply_cell, but the terms are face
+// fluxes instead of the large d x and G x_nb products, so the rounding floor
+// of a residual b - A x drops from ~eps * max|G x| per cell (about 2e-10 of
+// ||P|| at 1024^2, where the 250 um copper planes have G ~ 0.1 W/K) to
+// ~eps * |G dx|. Used for residuals that decide convergence.
+template <class G>
+__device__ __forceinline__ double flux_apply_cell(const Geo& g, const G* __restrict__ gE, const G* __restrict__ gN,
+                                                  const G* __restrict__ gU, const G* __restrict__ gB,
+                                                  const G* __restrict__ cap, const double* __restrict__ x,
+                                                  const Col& c, int l, double inv_dt) {
+    const long base = (long)l * g.plane;
+    const long id = base + c.c0;
+    const long idd = (l > 0) ? id - g.plane : id;
+    const long idu = (l < g.nz - 1) ? id + g.plane : id;
+    const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id], Gs = (double)gN[base + c.ps],
+                 Gn = (double)gN[id];
+    const double Gu = (double)gU[id], Gd = (l > 0) ? (double)gU[idd] : (double)gB[c.c0];
+    const double xc = x[id];
+    const double xw = x[base + c.pw], xe = x[base + c.pe], xs = x[base + c.ps], xn = x[base + c.pn];
+    const double xd = x[idd], xu = x[idu];
+    double f = 0.0;
+    if (c.hw) f += Gw * (xc - xw);
+    if (c.he) f += Ge * (xc - xe);
+    if (c.hs) f += Gs * (xc - xs);
+    if (c.hn) f += Gn * (xc - xn);
+    f += (l > 0) ? Gd * (xc - xd) : Gd * xc;         // below, or bottom ambient
+    f += (l < g.nz - 1) ? Gu * (xc - xu) : Gu * xc;  // above, or the heat sink
+    if (inv_dt != 0.0) f += (double)cap[id] * inv_dt * xc;
+    return f;
 }
 
 }  // namespace th

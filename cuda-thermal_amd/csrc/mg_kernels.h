@@ -1,0 +1,214 @@
+// mg_kernels.h - multigrid V-cycle kernels (src/mg3d.h, pcg3d_cpu.cpp:69-113
+// in what they compute; the launch design is this file's own).
+//
+//  * k_sweep: red-black column line smoother, one thread per column of the
+//    active colour; the column's Thomas solve runs in registers (NZ is the
+//    plane-count bucket), every load issued before the recurrence needs it.
+//  * k_restrict: one thread per (coarse cell, plane), so the coarse levels
+//    expose nz times more parallelism than a column walk.
+//  * k_tail: every level at or below 32 x 32 columns in a single workgroup,
+//    phases separated by barriers - one launch instead of ~5 per level for
+//    levels whose kernels would be pure launch latency.
+//  * The V-cycle needs no clearing pass: the first red sweep takes its black
+//    neighbours as zero (what the reference's memset-then-sweep computes) and
+//    the prolongation is folded into the first post-smoothing sweep, which
+//    adds the parent's coarse correction to each neighbour it reads.
+//  * Only 1/pivot is stored per cell; the upper factor is rebuilt as
+//    (-gU) * (1/pivot), the same product the reference stores.
+#pragma once
+#include "kernels.h"
+
+namespace th {
+
+enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
+constexpr int kTailThreads = 512;
+constexpr int kMaxTail = 8;
+constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the tail
+
+// Exact solve of column (i, j) with its lateral neighbours fixed
+// (mg3d.h:67-84), in place in x. MODE: see above. xc/gc: the coarse level
+// (SW_PROLONG only).
+template <class T, int NZ, int MODE>
+__device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__ gE, const T* __restrict__ gN,
+                                             const T* __restrict__ gU, const T* __restrict__ finv,
+                                             const T* __restrict__ b, T* __restrict__ x, int i, int j,
+                                             const Geo& gc, const T* __restrict__ xc) {
+    const Col c = make_col(g, i, j);
+    int qw = 0, qe = 0, qs = 0, qn = 0;
+    if (MODE == SW_PROLONG) {  // coarse parents of the 4 neighbours
+        qw = gc.pos((c.hw ? i - 1 : i) >> 1, j >> 1);
+        qe = gc.pos((c.he ? i + 1 : i) >> 1, j >> 1);
+        qs = gc.pos(i >> 1, (c.hs ? j - 1 : j) >> 1);
+        qn = gc.pos(i >> 1, (c.hn ? j + 1 : j) >> 1);
+    }
+    T xs[NZ], fcp[NZ];
+    T prev = 0, gd = 0;
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < g.nz) {
+            const long base = (long)l * g.plane;
+            const long id = base + c.c0;
+            T s = b[id];
+            if (MODE != SW_ZERO) {
+                T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
+                if (MODE == SW_PROLONG) {
+                    const long cb = (long)l * gc.plane;
+                    xw = xw + xc[cb + qw];
+                    xe = xe + xc[cb + qe];
+                    xsv = xsv + xc[cb + qs];
+                    xn = xn + xc[cb + qn];
+                }
+                const T Gw = gE[base + c.pw], Ge = gE[id], Gs = gN[base + c.ps], Gn = gN[id];
+                T ls = 0;
+                if (c.hw) ls += Gw * xw;
+                if (c.he) ls += Ge * xe;
+                if (c.hs) ls += Gs * xsv;
+                if (c.hn) ls += Gn * xn;
+                s = s + ls;
+            }
+            const T lower = (l > 0) ? -gd : T(0);
+            const T inv = finv[id];
+            prev = (s - lower * prev) * inv;
+            xs[l] = prev;
+            const T gu = gU[id];
+            fcp[l] = (l < g.nz - 1) ? (-gu) * inv : T(0);
+            gd = gu;
+        }
+    }
+#pragma unroll
+    for (int l = NZ - 2; l >= 0; --l)
+        if (l < g.nz - 1) xs[l] = xs[l] - fcp[l] * xs[l + 1];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l)
+        if (l < g.nz) x[(long)l * g.plane + c.c0] = xs[l];
+}
+
+// Coarse b at (I, J, l): sum over the (up to 4) children, row-major, of the
+// fine residual b - A x (pcg3d_cpu.cpp:87-100).
+template <class T>
+__device__ __forceinline__ void restrict_cell(const Geo& f, const Ops<T>& o, const T* __restrict__ fb,
+                                              const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb, int I,
+                                              int J, int l, T inv_dt) {
+    const long base = (long)l * f.plane;
+    T r[4];
+    bool ok[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+        ok[k] = i < f.nx && j < f.ny;
+        const Col c = make_col(f, ok[k] ? i : 2 * I, ok[k] ? j : 2 * J);
+        r[k] = fb[base + c.c0] - apply_cell<T>(f, o.gE, o.gN, o.gU, o.gB, o.cap, fx, c, l, inv_dt);
+    }
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (ok[k]) sum += r[k];
+    cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+}
+
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_sweep(Geo g, Ops<T> o, const T* __restrict__ finv,
+                                                const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc,
+                                                const T* __restrict__ xc, const State* __restrict__ st) {
+    if (st->done) return;
+    const int t = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    if (j >= g.ny) return;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= g.nx) return;
+    sweep_column<T, NZ, MODE>(g, o.gE, o.gN, o.gU, finv, b, x, i, j, gc, xc);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict(Geo f, Ops<T> o, const T* __restrict__ fb,
+                                                   const T* __restrict__ fx, Geo cg, T* __restrict__ cb,
+                                                   const State* __restrict__ st) {
+    if (st->done) return;
+    const int I = blockIdx.x * kColsX + threadIdx.x;
+    const int J = blockIdx.y * kColsY + threadIdx.y;
+    const int l = blockIdx.z;
+    if (I >= cg.nx || J >= cg.ny) return;
+    restrict_cell<T>(f, o, fb, fx, cg, cb, I, J, l, (T)st->inv_dt);
+}
+
+// Column factors (mg3d.h:46-61) for every column of a level: 1/pivot.
+template <class T>
+__global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__ finv, const State* __restrict__ st) {
+    const int q = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    int i;
+    if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
+    const Col c = make_col(g, i, j);
+    const T inv_dt = (T)st->inv_dt;
+    T cp_prev = 0;
+    for (int l = 0; l < g.nz; ++l) {
+        const long base = (long)l * g.plane, id = base + c.c0;
+        T d = 0;
+        if (c.hw) d += o.gE[base + c.pw];
+        if (c.he) d += o.gE[id];
+        if (c.hs) d += o.gN[base + c.ps];
+        if (c.hn) d += o.gN[id];
+        d += o.gU[id];
+        d += (l > 0) ? o.gU[id - g.plane] : o.gB[c.c0];
+        d = d + o.cap[id] * inv_dt;
+        const T lower = (l > 0) ? -o.gU[id - g.plane] : T(0);
+        const T upper = (l < g.nz - 1) ? -o.gU[id] : T(0);
+        const T inv = T(1) / (d - lower * cp_prev);
+        finv[id] = inv;
+        cp_prev = upper * inv;
+    }
+}
+
+// ---- coarse tail: the lowest levels of the V-cycle in one workgroup --------
+
+template <class T>
+struct TailLevel {
+    Geo g;
+    Ops<T> o;
+    const T* finv;
+    T* x;
+    T* b;
+};
+template <class T>
+struct Tail {
+    int n;  // levels in the tail; L[n-1] is the single column
+    TailLevel<T> L[kMaxTail];
+};
+
+template <class T, int NZ, int MODE>
+__device__ __forceinline__ void tail_sweep(const TailLevel<T>& L, int colour, const TailLevel<T>& C) {
+    const int cols = L.g.hx * L.g.ny;
+    for (int q = threadIdx.x; q < cols; q += kTailThreads) {
+        const int t = q % L.g.hx, j = q / L.g.hx;
+        const int i = 2 * t + ((j + colour) & 1);
+        if (i < L.g.nx)
+            sweep_column<T, NZ, MODE>(L.g, L.o.gE, L.o.gN, L.o.gU, L.finv, L.b, L.x, i, j, C.g, C.x);
+    }
+    __syncthreads();
+}
+
+// vcycle(k) for the levels of the tail (pcg3d_cpu.cpp:69-113); its first
+// level's b has been restricted by the caller.
+template <class T, int NZ>
+__global__ __launch_bounds__(kTailThreads) void k_tail(Tail<T> t, const State* __restrict__ st) {
+    if (st->done) return;
+    const T inv_dt = (T)st->inv_dt;
+    for (int k = 0; k + 1 < t.n; ++k) {
+        const TailLevel<T>&L = t.L[k], &C = t.L[k + 1];
+        tail_sweep<T, NZ, SW_ZERO>(L, 0, C);
+        tail_sweep<T, NZ, SW_PLAIN>(L, 1, C);
+        const int cells = C.g.nx * C.g.ny * C.g.nz;
+        for (int q = threadIdx.x; q < cells; q += kTailThreads) {
+            const int I = q % C.g.nx, rest = q / C.g.nx;
+            restrict_cell<T>(L.g, L.o, L.b, L.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny, inv_dt);
+        }
+        __syncthreads();
+    }
+    tail_sweep<T, NZ, SW_ZERO>(t.L[t.n - 1], 0, t.L[t.n - 1]);
+    for (int k = t.n - 2; k >= 0; --k) {
+        tail_sweep<T, NZ, SW_PROLONG>(t.L[k], 1, t.L[k + 1]);
+        tail_sweep<T, NZ, SW_PLAIN>(t.L[k], 0, t.L[k + 1]);
+    }
+}
+
+}  // namespace th

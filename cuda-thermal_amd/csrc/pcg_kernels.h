@@ -1,0 +1,340 @@
+// pcg_kernels.h - fine-grid PCG kernels in FP64 (src/pcg3d_gpu.cu:90-264 and
+// pcg3d_cpu.cpp:126-185 in what they compute).
+//
+// Stencil kernels run one thread per column (both colours of a row slot) and
+// walk the planes with the vertical neighbours in registers. Every reduction
+// writes one partial per block; a k_finalize launch adds them in block order
+// and runs the epilogue (alpha, beta, convergence test) on the device.
+#pragma once
+#include "kernels.h"
+
+namespace th {
+
+enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2 };
+enum { UP_MG = 0, UP_MG_FLEX = 1, UP_JACOBI = 2 };
+enum {
+    FIN_RES = 0,
+    FIN_RES_JACOBI,
+    FIN_CHECK,
+    FIN_ALPHA,
+    FIN_UPDATE,
+    FIN_UPDATE_FLEX,
+    FIN_UPDATE_JACOBI,
+    FIN_DOT_FIRST,
+    FIN_DOT,
+    FIN_ENERGY
+};
+
+// Start (or restart) of a solve: r = b - (A + C/dt) x in flux form, with
+// b0 = (T) r for the V-cycle (MG) or z = p = r / diag (Jacobi); partials
+// r.r, b.b (and r.z). RES_CHECK: the true residual ||P - A x|| of the steady
+// operator, no writes.
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
+                                                   const double* __restrict__ x, double* __restrict__ r,
+                                                   T* __restrict__ b0, const double* __restrict__ invd,
+                                                   double* __restrict__ z, double* __restrict__ p,
+                                                   const State* __restrict__ st, double* __restrict__ part) {
+    if (MODE != RES_CHECK && st->done) return;
+    const int q = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    double v[3] = {0.0, 0.0, 0.0};  // rr, bb, rz
+    const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
+    int i;
+    if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
+        const Col c = make_col(g, i, j);
+        for (int l = 0; l < g.nz; ++l) {
+            const long id = (long)l * g.plane + c.c0;
+            const double bi = b[id];
+            const double ri = bi - flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, x, c, l, inv_dt);
+            v[0] += ri * ri;
+            v[1] += bi * bi;
+            if (MODE == RES_MG) {
+                r[id] = ri;
+                b0[id] = (T)ri;
+            } else if (MODE == RES_JACOBI) {
+                const double zi = ri * invd[id];
+                r[id] = ri;
+                z[id] = zi;
+                p[id] = zi;
+                v[2] += ri * zi;
+            }
+        }
+    }
+    block_partials<3>(v, part);
+}
+
+// Ap = (A + C/dt) p; partial p.Ap.
+template <class T, int NZ>
+__global__ __launch_bounds__(256) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
+                                                double* __restrict__ Ap, const State* __restrict__ st,
+                                                double* __restrict__ part) {
+    if (st->done) return;
+    const int q = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    double v[1] = {0.0};
+    const double inv_dt = st->inv_dt;
+    const T* __restrict__ gE = o.gE;
+    const T* __restrict__ gN = o.gN;
+    const T* __restrict__ gU = o.gU;
+    const T* __restrict__ cap = o.cap;
+    int i;
+    if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
+        const Col c = make_col(g, i, j);
+        double pb = 0.0, pc = p[c.c0], gdn = (double)o.gB[c.c0];
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) {
+            if (l < g.nz) {
+                const long base = (long)l * g.plane, id = base + c.c0;
+                const double pu = (l < g.nz - 1) ? p[id + g.plane] : 0.0;
+                const double gu = (double)gU[id];
+                const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id];
+                const double Gs = (double)gN[base + c.ps], Gn = (double)gN[id];
+                const double xw = p[base + c.pw], xe = p[base + c.pe], xs = p[base + c.ps], xn = p[base + c.pn];
+                double d = 0.0, s = 0.0;
+                if (c.hw) d += Gw;
+                if (c.he) d += Ge;
+                if (c.hs) d += Gs;
+                if (c.hn) d += Gn;
+                d += gu;
+                d += gdn;  // gU of the plane below, or gB on the bottom plane
+                if (inv_dt != 0.0) d = d + (double)cap[id] * inv_dt;
+                if (c.hw) s += Gw * xw;
+                if (c.he) s += Ge * xe;
+                if (c.hs) s += Gs * xs;
+                if (c.hn) s += Gn * xn;
+                if (l > 0) s += gdn * pb;
+                if (l < g.nz - 1) s += gu * pu;
+                const double a = d * pc - s;
+                Ap[id] = a;
+                v[0] += pc * a;
+                pb = pc;
+                pc = pu;
+                gdn = gu;
+            }
+        }
+    }
+    block_partials<1>(v, part);
+}
+
+// x += alpha p, r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or
+// z = r / diag (Jacobi); partials r.r and r.z_old (flexible beta: z_old is the
+// previous V-cycle output, still in x0) or r.z (Jacobi).
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_update(long n, const double* __restrict__ p, const double* __restrict__ Ap,
+                                                 double* __restrict__ x, double* __restrict__ r, T* __restrict__ b0,
+                                                 const T* __restrict__ zold, const double* __restrict__ invd,
+                                                 double* __restrict__ z, const State* __restrict__ st,
+                                                 double* __restrict__ part) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double v[2] = {0.0, 0.0};
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        x[k] += alpha * p[k];
+        const double ri = r[k] - alpha * Ap[k];
+        r[k] = ri;
+        v[0] += ri * ri;
+        if (MODE == UP_JACOBI) {
+            const double zi = ri * invd[k];
+            z[k] = zi;
+            v[1] += ri * zi;
+        } else {
+            if (MODE == UP_MG_FLEX) v[1] += ri * (double)zold[k];
+            b0[k] = (T)ri;
+        }
+    }
+    block_partials<2>(v, part);
+}
+
+// Partial r.z with z = the V-cycle output.
+template <class T>
+__global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
+                                              const State* __restrict__ st, double* __restrict__ part) {
+    if (st->done) return;
+    double v[1] = {0.0};
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) v[0] += r[k] * (double)z[k];
+    block_partials<1>(v, part);
+}
+
+// p = z + beta p (first: p = z).
+template <class Z, int FIRST>
+__global__ __launch_bounds__(256) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
+                                                  const State* __restrict__ st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        p[k] = FIRST ? (double)z[k] : (double)z[k] + beta * p[k];
+}
+
+// Adds the partials of the previous kernel and runs its epilogue.
+template <int NV, int EPI, unsigned KMAX = 0>
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restrict__ part, int nblk, State* st) {
+    if (EPI != FIN_CHECK && EPI != FIN_ENERGY && st->done) return;
+    double tot[NV];
+    sum_partials<NV, KMAX>(part, nblk, tot);
+    if (threadIdx.x != 0) return;
+    if (EPI == FIN_RES || EPI == FIN_RES_JACOBI) {  // pcg3d_cpu.cpp:141-149
+        st->bb = tot[1];
+        st->rr = tot[0];
+        st->rzo = 0.0;
+        if (tot[1] == 0.0) {  // b = 0: x = 0 is the answer
+            st->relres = 0.0;
+            st->done = 1;
+            st->converged = 1;
+            st->zero_x = 1;
+            return;
+        }
+        st->relres = sqrt(tot[0] / tot[1]);
+        if (st->relres < st->tol) {
+            st->done = 1;
+            st->converged = 1;
+        }
+        if (EPI == FIN_RES_JACOBI) st->rz = tot[2];
+    } else if (EPI == FIN_CHECK) {
+        st->chk_rr = tot[0];
+        st->chk_bb = tot[1];
+    } else if (EPI == FIN_ALPHA) {
+        st->pAp = tot[0];
+        st->alpha = st->rz / tot[0];
+    } else if (EPI == FIN_UPDATE || EPI == FIN_UPDATE_FLEX || EPI == FIN_UPDATE_JACOBI) {
+        st->rr = tot[0];
+        st->iters += 1;
+        st->relres = sqrt(tot[0] / st->bb);
+        if (st->relres < st->tol) {
+            st->done = 1;
+            st->converged = 1;
+        } else if (st->iters >= st->maxit) {
+            st->done = 1;
+        }
+        if (EPI == FIN_UPDATE_FLEX) st->rzo = tot[1];
+        if (EPI == FIN_UPDATE_JACOBI && !st->done) {
+            st->beta = tot[1] / st->rz;
+            st->rz = tot[1];
+        }
+    } else if (EPI == FIN_DOT_FIRST) {
+        st->rz = tot[0];
+    } else if (EPI == FIN_DOT) {  // Fletcher-Reeves, or flexible (r.z - r.z_old) / rz_old
+        st->beta = st->flexible ? (tot[0] - st->rzo) / st->rz : tot[0] / st->rz;
+        st->rz = tot[0];
+    } else if (EPI == FIN_ENERGY) {
+        st->e_heat = tot[0];
+        st->e_out = tot[1];
+        st->e_tmax = tot[2];
+    }
+}
+
+// Start of a solve: tolerance, iteration cap, cleared flags.
+__global__ void k_begin(State* st, double tol, int maxit) {
+    st->tol = tol;
+    st->maxit = maxit;
+    st->iters = 0;
+    st->done = 0;
+    st->converged = 0;
+    st->zero_x = 0;
+    st->relres = 0.0;
+}
+// Reopens a finished solve for a restart from the current x (iterations kept).
+__global__ void k_reopen(State* st) {
+    st->done = 0;
+    st->converged = 0;
+}
+__global__ void k_set_inv_dt(State* st, double inv_dt) { st->inv_dt = inv_dt; }
+__global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
+
+// x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).
+__global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
+    if (!st->zero_x) return;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = 0.0;
+}
+
+// Jacobi: 1 / diag of (A + C/dt) (pcg3d_cpu.cpp:41-48).
+template <class T>
+__global__ __launch_bounds__(256) void k_invdiag(Geo g, Ops<T> o, double* __restrict__ invd, const State* st) {
+    const int q = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    int i;
+    if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
+    const Col c = make_col(g, i, j);
+    const double inv_dt = st->inv_dt;
+    for (int l = 0; l < g.nz; ++l) {
+        const long base = (long)l * g.plane, id = base + c.c0;
+        double d = 0;
+        if (c.hw) d += (double)o.gE[base + c.pw];
+        if (c.he) d += (double)o.gE[id];
+        if (c.hs) d += (double)o.gN[base + c.ps];
+        if (c.hn) d += (double)o.gN[id];
+        d += (double)o.gU[id];
+        d += (l > 0) ? (double)o.gU[id - g.plane] : (double)o.gB[c.c0];
+        d = d + (double)o.cap[id] * inv_dt;
+        invd[id] = 1.0 / d;
+    }
+}
+
+// b = P + C/dt x (pcg3d_gpu.cu:104-109).
+template <class T>
+__global__ __launch_bounds__(256) void k_rhs_transient(long n, const T* __restrict__ cap, const double* __restrict__ P,
+                                                        const double* __restrict__ x, double* __restrict__ b,
+                                                        const State* __restrict__ st) {
+    const double inv_dt = st->inv_dt;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        b[k] = P[k] + (double)cap[k] * inv_dt * x[k];
+}
+
+// Energy terms of the current x: sum C x (heat content), heat out through the
+// top and bottom (stack3d.h:254-263), and max x over cells (not padding).
+template <class T>
+__global__ __launch_bounds__(256) void k_energy(Geo g, Ops<T> o, const double* __restrict__ x,
+                                                 double* __restrict__ part) {
+    double v[3] = {0.0, 0.0, -1.0e300};
+    const long n = g.cells(), top = (long)(g.nz - 1) * g.plane;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double xv = x[k];
+        const long q = k % g.plane;
+        int i;
+        const bool cell = slot_column(g, (int)(q % (2 * g.hx)), (int)(q / (2 * g.hx)), i);
+        v[0] += (double)o.cap[k] * xv;
+        if (k >= top) v[1] += (double)o.gU[k] * xv;
+        if (k < g.plane) v[1] += (double)o.gB[k] * xv;
+        if (cell) v[2] = fmax(v[2], xv);
+    }
+    block_partials<3, 4u>(v, part);
+}
+
+// Natural <-> split layout. Padding slots of `out` are left untouched
+// (they were cleared at allocation).
+template <class S, class D>
+__global__ void k_to_split(Geo g, int planes, const S* __restrict__ in, D* __restrict__ out) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)g.nx * g.ny * planes;
+    if (k >= n) return;
+    const int i = (int)(k % g.nx);
+    const long rest = k / g.nx;
+    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
+    out[(long)l * g.plane + g.pos(i, j)] = (D)in[k];
+}
+__global__ void k_to_natural(Geo g, const double* __restrict__ in, double* __restrict__ out) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)g.nx * g.ny * g.nz;
+    if (k >= n) return;
+    const int i = (int)(k % g.nx);
+    const long rest = k / g.nx;
+    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
+    out[k] = in[(long)l * g.plane + g.pos(i, j)];
+}
+template <class D, class S>
+__global__ void k_cast(long n, const S* __restrict__ in, D* __restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = (D)in[k];
+}
+__global__ void k_mul(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = a[k] * b[k];
+}
+
+}  // namespace th

@@ -30,468 +30,11 @@
 
 #include "kernels.h"
 #include "levels.h"
+#include "mg_kernels.h"
 #include "model.h"
+#include "pcg_kernels.h"
 
 namespace th {
-
-// ===========================================================================
-// Kernels
-// ===========================================================================
-
-enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
-
-// Red-black column line smoother (mg3d.h:67-84): solves every column of one
-// colour exactly with its lateral neighbours held fixed, using the
-// precomputed 1/pivot. SW_ZERO: neighbours taken as zero (first sweep of a
-// V-cycle on a cleared level). SW_PROLONG: each neighbour reads
-// x + x_coarse(parent) (prolongation folded in).
-template <class T, int NZ, int MODE>
-__global__ __launch_bounds__(256) void k_sweep(Geo g, Ops<T> o, const T* __restrict__ finv,
-                                                const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc,
-                                                const T* __restrict__ xc, const State* __restrict__ st) {
-    if (st->done) return;
-    const int t = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    if (j >= g.ny) return;
-    const int i = 2 * t + ((j + colour) & 1);
-    if (i >= g.nx) return;
-    const Col c = make_col(g, i, j);
-    int qw = 0, qe = 0, qs = 0, qn = 0;
-    if (MODE == SW_PROLONG) {  // parents of the 4 neighbours on the coarse level
-        qw = c.hw ? gc.pos((i - 1) >> 1, j >> 1) : 0;
-        qe = c.he ? gc.pos((i + 1) >> 1, j >> 1) : 0;
-        qs = c.hs ? gc.pos(i >> 1, (j - 1) >> 1) : 0;
-        qn = c.hn ? gc.pos(i >> 1, (j + 1) >> 1) : 0;
-    }
-    T xs[NZ], fcp[NZ];
-    T prev = 0, gd = 0;
-#pragma unroll
-    for (int l = 0; l < NZ; ++l) {
-        if (l < g.nz) {
-            const long base = (long)l * g.plane;
-            const long id = base + c.c0;
-            T s = b[id];
-            if (MODE != SW_ZERO) {
-                const long cb = (long)l * gc.plane;
-                T ls = 0;
-                if (c.hw) {
-                    T xv = x[base + c.pw];
-                    if (MODE == SW_PROLONG) xv = xv + xc[cb + qw];
-                    ls += o.gE[base + c.pw] * xv;
-                }
-                if (c.he) {
-                    T xv = x[base + c.pe];
-                    if (MODE == SW_PROLONG) xv = xv + xc[cb + qe];
-                    ls += o.gE[id] * xv;
-                }
-                if (c.hs) {
-                    T xv = x[base + c.ps];
-                    if (MODE == SW_PROLONG) xv = xv + xc[cb + qs];
-                    ls += o.gN[base + c.ps] * xv;
-                }
-                if (c.hn) {
-                    T xv = x[base + c.pn];
-                    if (MODE == SW_PROLONG) xv = xv + xc[cb + qn];
-                    ls += o.gN[id] * xv;
-                }
-                s = s + ls;
-            }
-            const T lower = (l > 0) ? -gd : T(0);
-            const T inv = finv[id];
-            prev = (s - lower * prev) * inv;
-            xs[l] = prev;
-            const T gu = o.gU[id];
-            fcp[l] = (l < g.nz - 1) ? (-gu) * inv : T(0);
-            gd = gu;
-        }
-    }
-#pragma unroll
-    for (int l = NZ - 2; l >= 0; --l)
-        if (l < g.nz - 1) xs[l] = xs[l] - fcp[l] * xs[l + 1];
-#pragma unroll
-    for (int l = 0; l < NZ; ++l)
-        if (l < g.nz) x[(long)l * g.plane + c.c0] = xs[l];
-}
-
-// Column factors (mg3d.h:46-61) for every column of a level: 1/pivot.
-template <class T>
-__global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__ finv, const State* __restrict__ st) {
-    const int q = blockIdx.x * kColsX + threadIdx.x;  // slot in the row (both colours)
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    if (j >= g.ny || q >= 2 * g.hx) return;
-    const int colour = q >= g.hx ? 1 : 0, t = q - colour * g.hx;
-    const int i = 2 * t + ((j + colour) & 1);
-    if (i >= g.nx) return;
-    const Col c = make_col(g, i, j);
-    const T inv_dt = (T)st->inv_dt;
-    T cp_prev = 0;
-    for (int l = 0; l < g.nz; ++l) {
-        const long base = (long)l * g.plane, id = base + c.c0;
-        T d = 0;
-        if (c.hw) d += o.gE[base + c.pw];
-        if (c.he) d += o.gE[id];
-        if (c.hs) d += o.gN[base + c.ps];
-        if (c.hn) d += o.gN[id];
-        d += o.gU[id];
-        d += (l > 0) ? o.gU[id - g.plane] : o.gB[c.c0];
-        d = d + o.cap[id] * inv_dt;
-        const T lower = (l > 0) ? -o.gU[id - g.plane] : T(0);
-        const T upper = (l < g.nz - 1) ? -o.gU[id] : T(0);
-        const T inv = T(1) / (d - lower * cp_prev);
-        finv[id] = inv;
-        cp_prev = upper * inv;
-    }
-}
-
-// Restriction (pcg3d_cpu.cpp:87-100): coarse b = sum over the (up to 4)
-// children of the fine residual b - A x, children in row-major order. One
-// thread per coarse column; fixed summation order.
-template <class T>
-__global__ __launch_bounds__(256) void k_restrict(Geo f, Ops<T> o, const T* __restrict__ fb,
-                                                   const T* __restrict__ fx, Geo cg, T* __restrict__ cb,
-                                                   const State* __restrict__ st) {
-    if (st->done) return;
-    const int I = blockIdx.x * kColsX + threadIdx.x;
-    const int J = blockIdx.y * kColsY + threadIdx.y;
-    if (I >= cg.nx || J >= cg.ny) return;
-    const T inv_dt = (T)st->inv_dt;
-    // children in row-major order: (2I, 2J), (2I+1, 2J), (2I, 2J+1), (2I+1, 2J+1)
-    Col ch[4];
-    bool ok[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
-        ok[k] = i < f.nx && j < f.ny;
-        ch[k] = make_col(f, ok[k] ? i : 2 * I, ok[k] ? j : 2 * J);
-    }
-    const int cpos = cg.pos(I, J);
-    for (int l = 0; l < f.nz; ++l) {
-        const long base = (long)l * f.plane;
-        T sum = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (ok[k]) sum += fb[base + ch[k].c0] - apply_cell<T>(f, o, fx, ch[k], l, inv_dt);
-        cb[(long)l * cg.plane + cpos] = sum;
-    }
-}
-
-// ---- fine-grid PCG kernels (FP64) ------------------------------------------
-
-enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2 };
-
-// Start of a solve: r = b - (A + C/dt) x, with b0 = (T) r for the V-cycle
-// (MG) or z = p = r / diag (Jacobi); sums r.r and b.b (and r.z).
-// RES_CHECK: the true residual ||P - A x|| of the steady operator, no writes.
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
-                                                   const double* __restrict__ x, double* __restrict__ r,
-                                                   T* __restrict__ b0, const double* __restrict__ invd,
-                                                   double* __restrict__ z, double* __restrict__ p, State* st,
-                                                   double* part) {
-    if (MODE != RES_CHECK && st->done) return;
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    double v[3] = {0.0, 0.0, 0.0};  // rr, bb, rz
-    const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
-    if (j < g.ny && q < 2 * g.hx) {
-        const int colour = q >= g.hx ? 1 : 0, t = q - colour * g.hx;
-        const int i = 2 * t + ((j + colour) & 1);
-        if (i < g.nx) {
-            const Col c = make_col(g, i, j);
-            for (int l = 0; l < g.nz; ++l) {
-                const long id = (long)l * g.plane + c.c0;
-                const double bi = b[id];
-                const double ri = bi - apply_cell<double>(g, o, x, c, l, inv_dt);
-                v[0] += ri * ri;
-                v[1] += bi * bi;
-                if (MODE == RES_MG) {
-                    r[id] = ri;
-                    b0[id] = (T)ri;
-                } else if (MODE == RES_JACOBI) {
-                    const double zi = ri * invd[id];
-                    r[id] = ri;
-                    z[id] = zi;
-                    p[id] = zi;
-                    v[2] += ri * zi;
-                }
-            }
-        }
-    }
-    double tot[3];
-    if (grid_reduce<3>(v, part, &st->ticket[MODE == RES_CHECK ? TK_CHECK : TK_RES], tot) &&
-        threadIdx.x == 0 && threadIdx.y == 0) {
-        if (MODE == RES_CHECK) {
-            st->chk_rr = tot[0];
-            st->chk_bb = tot[1];
-            return;
-        }
-        st->bb = tot[1];
-        st->rr = tot[0];
-        st->iters = 0;
-        st->rzo = 0.0;
-        if (tot[1] == 0.0) {  // b = 0: x = 0 is the answer (pcg3d_cpu.cpp:146)
-            st->relres = 0.0;
-            st->done = 1;
-            st->converged = 1;
-            st->zero_x = 1;
-            return;
-        }
-        st->relres = sqrt(tot[0] / tot[1]);
-        if (st->relres < st->tol) {
-            st->done = 1;
-            st->converged = 1;
-        }
-        if (MODE == RES_JACOBI) st->rz = tot[2];
-    }
-}
-
-// Ap = (A + C/dt) p and p.Ap; epilogue: alpha = rz / pAp.
-template <class T, int NZ>
-__global__ __launch_bounds__(256) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
-                                                double* __restrict__ Ap, State* st, double* part) {
-    if (st->done) return;
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    double v[1] = {0.0};
-    const double inv_dt = st->inv_dt;
-    if (j < g.ny && q < 2 * g.hx) {
-        const int colour = q >= g.hx ? 1 : 0, t = q - colour * g.hx;
-        const int i = 2 * t + ((j + colour) & 1);
-        if (i < g.nx) {
-            const Col c = make_col(g, i, j);
-            double pb = 0.0, pc = p[c.c0], gdn = 0.0;
-#pragma unroll
-            for (int l = 0; l < NZ; ++l) {
-                if (l < g.nz) {
-                    const long base = (long)l * g.plane, id = base + c.c0;
-                    const double pu = (l < g.nz - 1) ? p[id + g.plane] : 0.0;
-                    const double gu = (double)o.gU[id];
-                    double d = 0.0, s = 0.0;
-                    double gw = 0, ge = 0, gs = 0, gn = 0;
-                    if (c.hw) gw = (double)o.gE[base + c.pw];
-                    if (c.he) ge = (double)o.gE[id];
-                    if (c.hs) gs = (double)o.gN[base + c.ps];
-                    if (c.hn) gn = (double)o.gN[id];
-                    if (c.hw) d += gw;
-                    if (c.he) d += ge;
-                    if (c.hs) d += gs;
-                    if (c.hn) d += gn;
-                    d += gu;
-                    d += (l > 0) ? gdn : (double)o.gB[c.c0];
-                    if (inv_dt != 0.0) d = d + (double)o.cap[id] * inv_dt;
-                    if (c.hw) s += gw * p[base + c.pw];
-                    if (c.he) s += ge * p[base + c.pe];
-                    if (c.hs) s += gs * p[base + c.ps];
-                    if (c.hn) s += gn * p[base + c.pn];
-                    if (l > 0) s += gdn * pb;
-                    if (l < g.nz - 1) s += gu * pu;
-                    const double a = d * pc - s;
-                    Ap[id] = a;
-                    v[0] += pc * a;
-                    pb = pc;
-                    pc = pu;
-                    gdn = gu;
-                }
-            }
-        }
-    }
-    double tot[1];
-    if (grid_reduce<1>(v, part, &st->ticket[TK_APPLY], tot) && threadIdx.x == 0 && threadIdx.y == 0) {
-        st->pAp = tot[0];
-        st->alpha = st->rz / tot[0];
-    }
-}
-
-enum { UP_MG = 0, UP_MG_FLEX = 1, UP_JACOBI = 2 };
-
-// x += alpha p, r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or
-// z = r / diag (Jacobi); sums r.r (and r.z_old for the flexible beta, where
-// z_old is the previous V-cycle output still held in x0, or r.z for Jacobi).
-// Epilogue: convergence test on ||r|| / ||b||, iteration count, Jacobi beta.
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_update(long n, const double* __restrict__ p, const double* __restrict__ Ap,
-                                                 double* __restrict__ x, double* __restrict__ r, T* __restrict__ b0,
-                                                 const T* __restrict__ zold, const double* __restrict__ invd,
-                                                 double* __restrict__ z, State* st, double* part) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double v[2] = {0.0, 0.0};
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        x[k] += alpha * p[k];
-        const double ri = r[k] - alpha * Ap[k];
-        r[k] = ri;
-        v[0] += ri * ri;
-        if (MODE == UP_JACOBI) {
-            const double zi = ri * invd[k];
-            z[k] = zi;
-            v[1] += ri * zi;
-        } else {
-            if (MODE == UP_MG_FLEX) v[1] += ri * (double)zold[k];
-            b0[k] = (T)ri;
-        }
-    }
-    double tot[2];
-    if (grid_reduce<2>(v, part, &st->ticket[TK_UPDATE], tot) && threadIdx.x == 0) {
-        st->rr = tot[0];
-        st->iters += 1;
-        st->relres = sqrt(tot[0] / st->bb);
-        if (st->relres < st->tol) {
-            st->done = 1;
-            st->converged = 1;
-        } else if (st->iters >= st->maxit) {
-            st->done = 1;
-        }
-        if (MODE == UP_MG_FLEX) st->rzo = tot[1];
-        if (MODE == UP_JACOBI && !st->done) {
-            st->beta = tot[1] / st->rz;
-            st->rz = tot[1];
-        }
-    }
-}
-
-// r.z with z = the V-cycle output; epilogue: beta and rz (first: rz only).
-// Flexible beta: (r.z - r.z_old) / rz_old.
-template <class T, int FIRST>
-__global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
-                                              State* st, double* part) {
-    if (st->done) return;
-    double v[1] = {0.0};
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) v[0] += r[k] * (double)z[k];
-    double tot[1];
-    if (grid_reduce<1>(v, part, &st->ticket[TK_DOT], tot) && threadIdx.x == 0) {
-        if (!FIRST) st->beta = st->flexible ? (tot[0] - st->rzo) / st->rz : tot[0] / st->rz;
-        st->rz = tot[0];
-    }
-}
-
-// p = z + beta p (first: p = z).
-template <class Z, int FIRST>
-__global__ __launch_bounds__(256) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
-                                                  const State* __restrict__ st) {
-    if (st->done) return;
-    const double beta = st->beta;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
-        p[k] = FIRST ? (double)z[k] : (double)z[k] + beta * p[k];
-}
-
-// Element-wise helpers for the test entry points.
-template <class D, class S>
-__global__ void k_cast(long n, const S* __restrict__ in, D* __restrict__ out) {
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = (D)in[k];
-}
-__global__ void k_mul(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = a[k] * b[k];
-}
-
-// Start of a solve: tolerance, iteration cap, cleared flags.
-__global__ void k_begin(State* st, double tol, int maxit) {
-    st->tol = tol;
-    st->maxit = maxit;
-    st->iters = 0;
-    st->done = 0;
-    st->converged = 0;
-    st->zero_x = 0;
-    st->relres = 0.0;
-}
-__global__ void k_set_inv_dt(State* st, double inv_dt) { st->inv_dt = inv_dt; }
-
-// x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).
-__global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
-    if (!st->zero_x) return;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = 0.0;
-}
-
-// Jacobi: 1 / diag of (A + C/dt) (pcg3d_cpu.cpp:41-48).
-template <class T>
-__global__ __launch_bounds__(256) void k_invdiag(Geo g, Ops<T> o, double* __restrict__ invd, const State* st) {
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    if (j >= g.ny || q >= 2 * g.hx) return;
-    const int colour = q >= g.hx ? 1 : 0, t = q - colour * g.hx;
-    const int i = 2 * t + ((j + colour) & 1);
-    if (i >= g.nx) return;
-    const Col c = make_col(g, i, j);
-    const double inv_dt = st->inv_dt;
-    for (int l = 0; l < g.nz; ++l) {
-        const long base = (long)l * g.plane, id = base + c.c0;
-        double d = 0;
-        if (c.hw) d += (double)o.gE[base + c.pw];
-        if (c.he) d += (double)o.gE[id];
-        if (c.hs) d += (double)o.gN[base + c.ps];
-        if (c.hn) d += (double)o.gN[id];
-        d += (double)o.gU[id];
-        d += (l > 0) ? (double)o.gU[id - g.plane] : (double)o.gB[c.c0];
-        d = d + (double)o.cap[id] * inv_dt;
-        invd[id] = 1.0 / d;
-    }
-}
-
-// b = P + C/dt x (pcg3d_gpu.cu:104-109).
-template <class T>
-__global__ __launch_bounds__(256) void k_rhs_transient(long n, const T* __restrict__ cap, const double* __restrict__ P,
-                                                        const double* __restrict__ x, double* __restrict__ b,
-                                                        const State* __restrict__ st) {
-    const double inv_dt = st->inv_dt;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
-        b[k] = P[k] + (double)cap[k] * inv_dt * x[k];
-}
-
-// Energy terms of the current x: sum C x (heat content), heat out through
-// the top and bottom (stack3d.h:254-263), and max x.
-template <class T>
-__global__ __launch_bounds__(256) void k_energy(Geo g, Ops<T> o, const double* __restrict__ x, State* st,
-                                                 double* part) {
-    double v[3] = {0.0, 0.0, -1.0e300};
-    const long n = g.cells(), top = (long)(g.nz - 1) * g.plane;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const double xv = x[k];
-        // padding slots (odd nx) hold zeros and are not cells: skip them for the max
-        const long q = k % g.plane;
-        const int row = (int)(q / (2 * g.hx)), slot = (int)(q % (2 * g.hx));
-        const int colour = slot >= g.hx ? 1 : 0;
-        const bool cell = 2 * (slot - colour * g.hx) + ((row + colour) & 1) < g.nx;
-        v[0] += (double)o.cap[k] * xv;
-        if (k >= top) v[1] += (double)o.gU[k] * xv;
-        if (k < g.plane) v[1] += (double)o.gB[k] * xv;
-        if (cell) v[2] = fmax(v[2], xv);
-    }
-    double tot[3];
-    if (grid_reduce<3, 4u>(v, part, &st->ticket[TK_ENERGY], tot) && threadIdx.x == 0) {
-        st->e_heat = tot[0];
-        st->e_out = tot[1];
-        st->e_tmax = tot[2];
-    }
-}
-
-// Natural <-> split layout. Padding slots of `out` are left untouched
-// (they were cleared at allocation).
-template <class S, class D>
-__global__ void k_to_split(Geo g, int planes, const S* __restrict__ in, D* __restrict__ out) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)g.nx * g.ny * planes;
-    if (k >= n) return;
-    const int i = (int)(k % g.nx);
-    const long rest = k / g.nx;
-    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
-    out[(long)l * g.plane + g.pos(i, j)] = (D)in[k];
-}
-__global__ void k_to_natural(Geo g, const double* __restrict__ in, double* __restrict__ out) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)g.nx * g.ny * g.nz;
-    if (k >= n) return;
-    const int i = (int)(k % g.nx);
-    const long rest = k / g.nx;
-    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
-    out[k] = in[(long)l * g.plane + g.pos(i, j)];
-}
 
 // ===========================================================================
 // Host side
@@ -538,6 +81,7 @@ struct SolverBase {
     virtual hipStream_t stream() const = 0;
     virtual int precondition(const double* r, double* z) = 0;
     virtual int apply_operator(const double* x, double* y) = 0;
+    virtual int time_kernel(int which, int reps, double* avg_ms, double* bytes) = 0;
 };
 
 template <class T>
@@ -649,6 +193,12 @@ struct Solver final : SolverBase {
             if (!mg()) break;
         }
         if ((rc = dalloc(&part, maxblk * 4))) return rc;
+        // Levels with at most kTailCols columns (and at most kMaxTail of
+        // them) run in the single-workgroup tail kernel.
+        tail_k0 = L.size() - 1;
+        while (tail_k0 > 0 && (long)L[tail_k0 - 1].g.nx * L[tail_k0 - 1].g.ny <= kTailCols &&
+               L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
+            --tail_k0;
         if ((rc = upload_split<double, double>(g0, g0.nz, m.P, P))) return rc;
         if ((rc = set_dt(0.0))) return rc;
         HIPTRY(hipStreamSynchronize(s));
@@ -729,6 +279,15 @@ struct Solver final : SolverBase {
 
     // ---- launch sequences -------------------------------------------------
 
+    size_t tail_k0 = 0;  // first level handled by the single-workgroup tail
+
+    template <int NV, int EPI, unsigned KMAX = 0>
+    int finalize(unsigned nblk) {
+        k_finalize<NV, EPI, KMAX><<<1, kFinThreads, 0, s>>>(part, (int)nblk, st);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
     template <int MODE>
     int launch_sweep(size_t k, int colour) {
         const DevLevel<T>& d = L[k];
@@ -748,17 +307,42 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 
+    int launch_tail() {
+        Tail<T> t{};
+        t.n = (int)(L.size() - tail_k0);
+        for (int k = 0; k < t.n; ++k) {
+            const DevLevel<T>& d = L[tail_k0 + k];
+            t.L[k] = TailLevel<T>{d.g, d.ops(), d.finv, d.x, d.b};
+        }
+        const int nz = g0.nz;
+#define TH_TAIL(NZ) k_tail<T, NZ><<<1, kTailThreads, 0, s>>>(t, st)
+        if (nz <= 4) TH_TAIL(4);
+        else if (nz <= 8) TH_TAIL(8);
+        else if (nz <= 12) TH_TAIL(12);
+        else if (nz <= 16) TH_TAIL(16);
+        else if (nz <= 24) TH_TAIL(24);
+        else if (nz <= 32) TH_TAIL(32);
+        else if (nz <= 48) TH_TAIL(48);
+        else TH_TAIL(64);
+#undef TH_TAIL
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
     // One V-cycle on level k (pcg3d_cpu.cpp:69-113): pre-smooth red, black;
     // restrict; coarse V-cycle; post-smooth black (with the prolongation),
-    // red. Level k's x starts from zero.
+    // red. Level k's x starts from zero. Levels from tail_k0 down run in the
+    // single-workgroup tail kernel.
     int vcycle(size_t k) {
         int rc;
-        if (k + 1 == L.size()) return launch_sweep<SW_ZERO>(k, 0);  // one column: exact
+        if (k == tail_k0) return launch_tail();
         if ((rc = launch_sweep<SW_ZERO>(k, 0))) return rc;
         if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
         const DevLevel<T>& f = L[k];
         const DevLevel<T>& c = L[k + 1];
-        k_restrict<T><<<col_grid(c.g.nx, c.g.ny), col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
+        dim3 grid = col_grid(c.g.nx, c.g.ny);
+        grid.z = c.g.nz;
+        k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
         HIPTRY(hipGetLastError());
         if ((rc = vcycle(k + 1))) return rc;
         if ((rc = launch_sweep<SW_PROLONG>(k, 1))) return rc;
@@ -779,7 +363,7 @@ struct Solver final : SolverBase {
         else TH_APPLY(64);
 #undef TH_APPLY
         HIPTRY(hipGetLastError());
-        return TH_OK;
+        return finalize<1, FIN_ALPHA>(grid.x * grid.y);
     }
 
     // Start: residual, first preconditioner application, rz, p = z.
@@ -791,16 +375,19 @@ struct Solver final : SolverBase {
             k_residual<T, RES_MG><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
                                                                nullptr, st, part);
             HIPTRY(hipGetLastError());
+            if ((rc = finalize<3, FIN_RES>(grid.x * grid.y))) return rc;
             if ((rc = vcycle(0))) return rc;
-            k_dot<T, 1><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
+            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
             HIPTRY(hipGetLastError());
+            if ((rc = finalize<1, FIN_DOT_FIRST>(vb))) return rc;
             k_pupdate<T, 1><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st);
-        } else {
-            k_residual<T, RES_JACOBI><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
-                                                                   part);
+            HIPTRY(hipGetLastError());
+            return TH_OK;
         }
+        k_residual<T, RES_JACOBI><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
+                                                               part);
         HIPTRY(hipGetLastError());
-        return TH_OK;
+        return finalize<3, FIN_RES_JACOBI>(grid.x * grid.y);
     }
 
     // One PCG iteration: Ap and alpha; x, r update and the convergence test;
@@ -810,20 +397,26 @@ struct Solver final : SolverBase {
         int rc;
         if ((rc = launch_apply())) return rc;
         if (mg()) {
-            if (opt.beta == TH_BETA_FLEXIBLE)
+            if (opt.beta == TH_BETA_FLEXIBLE) {
                 k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, L[0].x, nullptr, nullptr,
                                                                    st, part);
-            else
+                HIPTRY(hipGetLastError());
+                if ((rc = finalize<2, FIN_UPDATE_FLEX>(vb))) return rc;
+            } else {
                 k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, nullptr, nullptr, nullptr, st,
                                                               part);
-            HIPTRY(hipGetLastError());
+                HIPTRY(hipGetLastError());
+                if ((rc = finalize<2, FIN_UPDATE>(vb))) return rc;
+            }
             if ((rc = vcycle(0))) return rc;
-            k_dot<T, 0><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
+            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
             HIPTRY(hipGetLastError());
+            if ((rc = finalize<1, FIN_DOT>(vb))) return rc;
             k_pupdate<T, 0><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st);
         } else {
             k_update<T, UP_JACOBI><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, nullptr, nullptr, invd, z, st, part);
             HIPTRY(hipGetLastError());
+            if ((rc = finalize<2, FIN_UPDATE_JACOBI>(vb))) return rc;
             k_pupdate<double, 0><<<vb, kVecThreads, 0, s>>>(n0, z, p, st);
         }
         HIPTRY(hipGetLastError());
@@ -836,26 +429,63 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 
-    int solve(double tol, int maxit, th_stats* out) override {
-        int rc;
-        // Flexible beta is a property of the solve; it rides in the state.
-        k_begin<<<1, 1, 0, s>>>(st, tol, maxit);
-        HIPTRY(hipGetLastError());
+    int run_start() {
         if (g_start) {
             HIPTRY(hipGraphLaunch(g_start, s));
-        } else if ((rc = enqueue_start())) {
-            return rc;
+            return TH_OK;
         }
-        if ((rc = fetch_state())) return rc;
-        // The host checks convergence after every iteration, like the
-        // reference; kernels of a finished solve return at once.
-        for (int it = 0; !h_st->done && it < maxit; ++it) {
-            if (g_iter) {
-                HIPTRY(hipGraphLaunch(g_iter, s));
-            } else if ((rc = enqueue_iter())) {
-                return rc;
+        return enqueue_start();
+    }
+    int run_iter() {
+        if (g_iter) {
+            HIPTRY(hipGraphLaunch(g_iter, s));
+            return TH_OK;
+        }
+        return enqueue_iter();
+    }
+
+    // PCG iterations until the device reports done. Iterations are queued in
+    // batches sized from the observed convergence rate, so the host waits
+    // once per batch rather than once per iteration; iterations queued past
+    // convergence return at once.
+    int iterate(int maxit) {
+        int rc;
+        double rate = 0.0;  // relres reduction per iteration, once known
+        while (!h_st->done && h_st->iters < maxit) {
+            int k = 1;
+            if (rate > 0.0 && rate < 1.0 && h_st->relres > 0.0) {
+                const double need = std::log(h_st->tol / h_st->relres) / std::log(rate);
+                k = (int)std::max(1.0, std::min(16.0, std::ceil(need) - 1.0));
             }
+            k = std::min(k, maxit - h_st->iters);
+            const int it0 = h_st->iters;
+            const double rel0 = h_st->relres;
+            for (int q = 0; q < k; ++q)
+                if ((rc = run_iter())) return rc;
             if ((rc = fetch_state())) return rc;
+            const int did = h_st->iters - it0;
+            if (did > 0 && rel0 > 0.0 && h_st->relres > 0.0) rate = std::pow(h_st->relres / rel0, 1.0 / did);
+            if (did == 0) break;  // nothing ran: the device is done
+        }
+        return TH_OK;
+    }
+
+    int solve(double tol, int maxit, th_stats* out) override {
+        int rc;
+        k_begin<<<1, 1, 0, s>>>(st, tol, maxit);
+        HIPTRY(hipGetLastError());
+        if ((rc = run_start()) || (rc = fetch_state())) return rc;
+        if (maxit > 0 && (rc = iterate(maxit))) return rc;
+        // Optional true-residual check: the recurrence ||r|| drifts from
+        // ||b - A x|| by rounding; restart from x with the true residual
+        // until that one is below tol too.
+        for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
+            k_reopen<<<1, 1, 0, s>>>(st);
+            HIPTRY(hipGetLastError());
+            if ((rc = run_start()) || (rc = fetch_state())) return rc;
+            if (h_st->done) break;  // true residual below tol
+            if (h_st->iters >= maxit) break;
+            if ((rc = iterate(maxit))) return rc;
         }
         k_finish<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, st);
         HIPTRY(hipGetLastError());
@@ -873,16 +503,17 @@ struct Solver final : SolverBase {
                                                               nullptr, nullptr, st, part);
         HIPTRY(hipGetLastError());
         int rc;
-        if ((rc = fetch_state())) return rc;
+        if ((rc = finalize<3, FIN_CHECK>(grid.x * grid.y)) || (rc = fetch_state())) return rc;
         *out = std::sqrt(h_st->chk_rr / h_st->chk_bb);
         return TH_OK;
     }
 
     int energy(double* heat, double* outq, double* tmax) override {
-        k_energy<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(g0, L[0].ops(), x, st, part);
+        const unsigned vb = vec_blocks(n0);
+        k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
         HIPTRY(hipGetLastError());
         int rc;
-        if ((rc = fetch_state())) return rc;
+        if ((rc = finalize<3, FIN_ENERGY, 4u>(vb)) || (rc = fetch_state())) return rc;
         if (heat) *heat = h_st->e_heat;
         if (outq) *outq = h_st->e_out;
         if (tmax) *tmax = h_st->e_tmax;
@@ -890,7 +521,7 @@ struct Solver final : SolverBase {
     }
 
     // Test entry points: one preconditioner application / one operator apply
-    // on host vectors. They use r and p as staging and leave x, b untouched.
+    // on host vectors. They use r, p and Ap as staging and leave x, b alone.
     int precondition(const double* host_r, double* host_z) override {
         int rc;
         if ((rc = upload_split<double, double>(g0, g0.nz, host_r, r))) return rc;
@@ -917,6 +548,76 @@ struct Solver final : SolverBase {
         if ((rc = launch_apply())) return rc;
         return download_natural(Ap, host_y);
     }
+    int time_kernel(int which, int reps, double* avg_ms, double* bytes) override {
+        if (reps < 1) return fail(TH_ERR_ARG, "reps must be >= 1");
+        if (which != TH_K_APPLY && which != TH_K_COPY && !mg())
+            return fail(TH_ERR_ARG, "multigrid kernels need a multigrid solver");
+        if (which == TH_K_RESTRICT && L.size() < 2) return fail(TH_ERR_ARG, "no coarse level");
+        const double N = (double)g0.nx * g0.ny * g0.nz, sz = (double)sizeof(T);
+        auto launch = [&]() -> int {
+            switch (which) {
+                case TH_K_SWEEP:
+                    return launch_sweep<SW_PLAIN>(0, 0);
+                case TH_K_APPLY: {
+                    const dim3 grid = col_grid(2 * g0.hx, g0.ny);
+                    const Ops<T> o = L[0].ops();
+#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, p, Ap, st, part)
+                    if (g0.nz <= 8) TH_APPLY(8);
+                    else if (g0.nz <= 12) TH_APPLY(12);
+                    else if (g0.nz <= 16) TH_APPLY(16);
+                    else if (g0.nz <= 32) TH_APPLY(32);
+                    else TH_APPLY(64);
+#undef TH_APPLY
+                    HIPTRY(hipGetLastError());
+                    return TH_OK;
+                }
+                case TH_K_RESTRICT: {
+                    const DevLevel<T>&f = L[0], &c = L[1];
+                    dim3 grid = col_grid(c.g.nx, c.g.ny);
+                    grid.z = c.g.nz;
+                    k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
+                    HIPTRY(hipGetLastError());
+                    return TH_OK;
+                }
+                case TH_K_COPY:
+                    k_cast<double, double><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, p, Ap);
+                    HIPTRY(hipGetLastError());
+                    return TH_OK;
+                case TH_K_VCYCLE:
+                    return vcycle(0);
+                default:
+                    return fail(TH_ERR_ARG, "unknown kernel id");
+            }
+        };
+        switch (which) {
+            case TH_K_SWEEP: *bytes = 4.5 * sz * N; break;                  // b, x(other), gE, gN, gU, 1/pivot; x
+            case TH_K_APPLY: *bytes = (16.0 + 3.0 * sz) * N; break;         // p, Ap; gE, gN, gU
+            case TH_K_RESTRICT: *bytes = 5.0 * sz * N + 0.25 * sz * N; break;  // b, x, gE, gN, gU; coarse b
+            case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
+            default: *bytes = 0.0;
+        }
+        k_reopen<<<1, 1, 0, s>>>(st);  // kernels return at once on a finished solve
+        HIPTRY(hipGetLastError());
+        int rc;
+        if ((rc = launch())) return rc;
+        hipEvent_t e0, e1;
+        HIPTRY(hipEventCreate(&e0));
+        HIPTRY(hipEventCreate(&e1));
+        hipError_t e = hipEventRecord(e0, s);
+        for (int k = 0; k < reps && e == hipSuccess; ++k)
+            if ((rc = launch())) break;
+        if (e == hipSuccess && !rc) e = hipEventRecord(e1, s);
+        if (e == hipSuccess && !rc) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("kernel timing: ") + hipGetErrorString(e));
+        *avg_ms = ms / reps;
+        return TH_OK;
+    }
+
     int download_natural(const double* dev, double* host) {
         k_to_natural<<<lin_blocks(nnat), 256, 0, s>>>(g0, dev, xnat);
         HIPTRY(hipGetLastError());
@@ -925,9 +626,6 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 };
-
-// The flexible flag lives in the device state; set once at creation.
-__global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
 
 }  // namespace th
 
@@ -948,6 +646,7 @@ void th_solver_default_opts(th_solver_opts* o) {
     o->beta = TH_BETA_FLEXIBLE;
     o->use_graphs = 1;
     o->device = 0;
+    o->true_residual = 0;
 }
 
 static int check_desc(const th_model_desc* m) {
@@ -1053,6 +752,11 @@ int th_solver_apply_operator(th_solver* s, const double* x, double* y) {
     TH_S(s);
     if (!x || !y) return fail(TH_ERR_ARG, "null argument");
     return s->impl->apply_operator(x, y);
+}
+int th_solver_time_kernel(th_solver* s, int which, int reps, double* avg_ms, double* bytes) {
+    TH_S(s);
+    if (!avg_ms || !bytes) return fail(TH_ERR_ARG, "null argument");
+    return s->impl->time_kernel(which, reps, avg_ms, bytes);
 }
 
 int th_steady_solve(const th_model_desc* m, const th_solver_opts* o, double tol, int maxit, double* T,

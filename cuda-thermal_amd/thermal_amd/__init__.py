@@ -49,7 +49,7 @@ class ModelDesc(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("precond", C.c_int), ("precision", C.c_int), ("beta", C.c_int), ("use_graphs", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("true_residual", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -120,6 +120,7 @@ SIGNATURES = {
     "th_solver_nlevels": (C.c_int, [_vp]),
     "th_solver_precondition": (C.c_int, [_vp, _dp, _dp]),
     "th_solver_apply_operator": (C.c_int, [_vp, _dp, _dp]),
+    "th_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "th_steady_solve": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(SolverOpts), C.c_double, C.c_int, _dp,
                                   C.POINTER(SolveResult)]),
     "th_transient_default_opts": (None, [C.POINTER(TransientOpts)]),
@@ -309,15 +310,16 @@ def desc_from_arrays(a):
     return d, keep
 
 
-def make_opts(precond="mg", precision="mixed", beta="flexible", graphs=True, device=0):
-    return SolverOpts(PRECOND[precond], PRECISION[precision], BETA[beta], int(graphs), device)
+def make_opts(precond="mg", precision="mixed", beta="flexible", graphs=True, device=0, true_residual=False):
+    return SolverOpts(PRECOND[precond], PRECISION[precision], BETA[beta], int(graphs), device, int(true_residual))
 
 
 class Solver:
     """GpuPcg's interface (pcg3d.h:115-163) over the C ABI. `model` is a
     Model or a dict of arrays (gE, gN, gU, gB, cap, P, nx, ny, nz, dx, dy)."""
 
-    def __init__(self, model, precond="mg", precision="mixed", beta="flexible", graphs=True, device=0):
+    def __init__(self, model, precond="mg", precision="mixed", beta="flexible", graphs=True, device=0,
+                 true_residual=False):
         if isinstance(model, Model):
             d, self._keep = model.desc(), model
             self.nx, self.ny, self.nz = model.nx, model.ny, model.nz
@@ -327,7 +329,7 @@ class Solver:
         self.n = self.nx * self.ny * self.nz
         self.h = None
         h = _vp()
-        opts = make_opts(precond, precision, beta, graphs, device)
+        opts = make_opts(precond, precision, beta, graphs, device, true_residual)
         _check(lib().th_solver_create(C.byref(d), C.byref(opts), C.byref(h)))
         self.h = h
 
@@ -392,6 +394,15 @@ class Solver:
         z = np.empty(self.n, np.float64)
         _check(lib().th_solver_precondition(self.h, _d(r), _d(z)))
         return z
+
+    KERNELS = {"sweep": 0, "apply": 1, "restrict": 2, "copy": 3, "vcycle": 4}
+
+    def time_kernel(self, which, reps=20):
+        """(average ms per launch, algorithmic bytes per launch) of one
+        hot-path kernel, timed with HIP events on the solver's stream."""
+        ms, nb = C.c_double(), C.c_double()
+        _check(lib().th_solver_time_kernel(self.h, self.KERNELS[which], reps, C.byref(ms), C.byref(nb)))
+        return ms.value, nb.value
 
     def apply(self, x):
         """y = (A + C/dt) x in FP64."""

@@ -111,6 +111,9 @@ typedef struct {
     int beta;       /* TH_BETA_*; the reference uses Fletcher-Reeves (pcg3d_cpu.cpp:179) */
     int use_graphs; /* replay iterations from hipGraphs (gpu_use_graphs, pcg3d.h:167) */
     int device;     /* HIP device ordinal */
+    int true_residual; /* 1: on convergence recompute b - A x in FP64 and restart
+                          from x while it is above tol (up to 4 restarts); 0: stop
+                          on the recurrence residual, as the reference does */
 } th_solver_opts;
 
 TH_API void th_solver_default_opts(th_solver_opts* o);
@@ -146,6 +149,19 @@ TH_API int th_solver_nlevels(th_solver* s);
 TH_API int th_solver_precondition(th_solver* s, const double* r, double* z);
 /* y = (A + C/dt) x in FP64 for the current dt (operator3d.h:109-112). */
 TH_API int th_solver_apply_operator(th_solver* s, const double* x, double* y);
+
+/* Measurement (no reference counterpart; the reference times kernels with
+ * Nsight only): average duration of `reps` back-to-back launches of one
+ * hot-path kernel on the solver's stream, timed with HIP events, and the
+ * kernel's algorithmic bytes per launch. Call after a solve (the kernels run
+ * on the solver's current vectors; x and b of the PCG are not modified).
+ *   TH_K_SWEEP    fine-level colour sweep (red columns, black neighbours)
+ *   TH_K_APPLY    fine-level FP64 operator apply Ap = (A + C/dt) p
+ *   TH_K_RESTRICT fine-to-first-coarse restriction
+ *   TH_K_COPY     streaming copy of one FP64 vector (bandwidth reference)
+ *   TH_K_VCYCLE   one whole V-cycle (bytes reported as 0) */
+enum { TH_K_SWEEP = 0, TH_K_APPLY = 1, TH_K_RESTRICT = 2, TH_K_COPY = 3, TH_K_VCYCLE = 4 };
+TH_API int th_solver_time_kernel(th_solver* s, int which, int reps, double* avg_ms, double* bytes);
 
 /* Steady solve from T = 0 with timing (gpu_pcg, pcg3d_gpu.cu:686-719). */
 typedef struct {

@@ -629,6 +629,33 @@ static double true_relres(const Op& o, const double* P, const double* T) {
     return std::sqrt(rr / bb);
 }
 
+// ||P - A T|| / ||P|| with A T evaluated in flux form (sum of G (T_c - T_nb)
+// over faces plus the ambient links): the same operator as `apply`, with a
+// rounding floor set by the temperature differences instead of T itself.
+// No reference counterpart; pinned against true_relres where both floors are
+// far below the residual, and by the exact 1D chain.
+static double true_relres_flux(const Op& o, const double* P, const double* T) {
+    double rr = 0, bb = 0;
+    const size_t sxy = o.plane();
+    for (int l = 0; l < o.nz; ++l)
+        for (int j = 0; j < o.ny; ++j)
+            for (int i = 0; i < o.nx; ++i) {
+                const size_t c = o.idx(i, j, l);
+                const double x = T[c];
+                double f = 0;
+                if (i > 0) f += (double)o.gE[c - 1] * (x - T[c - 1]);
+                if (i < o.nx - 1) f += (double)o.gE[c] * (x - T[c + 1]);
+                if (j > 0) f += (double)o.gN[c - o.nx] * (x - T[c - o.nx]);
+                if (j < o.ny - 1) f += (double)o.gN[c] * (x - T[c + o.nx]);
+                f += (l > 0) ? (double)o.gU[c - sxy] * (x - T[c - sxy]) : (double)o.gB[o.col(i, j)] * x;
+                f += (l < o.nz - 1) ? (double)o.gU[c] * (x - T[c + sxy]) : (double)o.gU[c] * x;
+                const double r = P[c] - f;
+                rr += r * r;
+                bb += P[c] * P[c];
+            }
+    return std::sqrt(rr / bb);
+}
+
 // stack3d.h:254-273
 static double heat_out(const Op& o, const double* T) {
     double q = 0;
@@ -758,6 +785,11 @@ double or_true_relres(int nx, int ny, int nz, const float* gE, const float* gN, 
                       const float* gB, const float* cap, const double* P, const double* T) {
     const Op o{nx, ny, nz, gE, gN, gU, gB, cap};
     return true_relres(o, P, T);
+}
+double or_true_relres_flux(int nx, int ny, int nz, const float* gE, const float* gN, const float* gU,
+                           const float* gB, const float* cap, const double* P, const double* T) {
+    const Op o{nx, ny, nz, gE, gN, gU, gB, cap};
+    return true_relres_flux(o, P, T);
 }
 double or_heat_out(int nx, int ny, int nz, const float* gU, const float* gB, const double* T) {
     const Op o{nx, ny, nz, nullptr, nullptr, gU, gB, nullptr};

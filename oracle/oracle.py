@@ -93,6 +93,8 @@ def lib():
         L.or_solver_level_factors.argtypes = [_vp, C.c_int, _dp, _dp]
         L.or_true_relres.argtypes = [C.c_int] * 3 + [_fp] * 5 + [_dp, _dp]
         L.or_true_relres.restype = C.c_double
+        L.or_true_relres_flux.argtypes = [C.c_int] * 3 + [_fp] * 5 + [_dp, _dp]
+        L.or_true_relres_flux.restype = C.c_double
         L.or_heat_out.argtypes = [C.c_int] * 3 + [_fp, _fp, _dp]
         L.or_heat_out.restype = C.c_double
         L.or_heat_stored.argtypes = [C.c_int] * 3 + [_fp, _dp, _dp, C.c_double]
@@ -307,6 +309,15 @@ def true_relres(arrays, T):
     P = np.ascontiguousarray(a["P"], np.float64)
     T = np.ascontiguousarray(T, np.float64)
     return lib().or_true_relres(*_op_args(a), *[_f(v) for v in keep], _d(P), _d(T))
+
+
+def true_relres_flux(arrays, T):
+    """||P - A T|| / ||P|| with A T in flux form (lower rounding floor)."""
+    a = arrays
+    keep = [np.ascontiguousarray(a[k], np.float32) for k in ("gE", "gN", "gU", "gB", "cap")]
+    P = np.ascontiguousarray(a["P"], np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    return lib().or_true_relres_flux(*_op_args(a), *[_f(v) for v in keep], _d(P), _d(T))
 
 
 def heat_out(arrays, T):

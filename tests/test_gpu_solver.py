@@ -181,3 +181,17 @@ def test_set_power_and_warm_start():
     x2 = s.get_x()
     assert st["converged"]
     assert np.abs(x2 - 2.0 * x1).max() < 1e-8 * x2.max()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_true_residual_restart(precision):
+    """With the true-residual check, the solve only reports convergence once
+    ||P - A T|| / ||P|| itself is below tol (the recurrence drifts by
+    rounding on large grids)."""
+    m = hotspot(512, at=(5e-3, 5e-3))
+    s = T.Solver(m, precision=precision, true_residual=True)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 200)
+    assert st["converged"]
+    assert s.true_relres() < 1e-10
