@@ -111,8 +111,8 @@ def kernels_only(n):
     s.set_x_zero()
     s.set_rhs_steady()
     s.solve(TOL, 500)
-    for which in ("sweep", "copy"):
-        ms, nb = s.time_kernel(which, 5)
+    for which in ("sweep", "apply", "restrict", "copy"):
+        ms, nb = s.time_kernel(which, 3)
         log(f"{which}: {ms * 1e3:.1f} us, {nb / 1e6:.1f} MB algorithmic, {nb / ms / 1e6:.0f} GB/s")
 
 

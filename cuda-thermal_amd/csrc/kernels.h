@@ -27,8 +27,9 @@ namespace th {
 
 constexpr int kColsX = 64;  // threads per block along a half-row (one wave)
 constexpr int kColsY = 4;   // rows per block
-constexpr int kVecThreads = 256;
-constexpr int kMaxVecBlocks = 1024;
+constexpr int kVecThreads = 1024;
+constexpr int kMaxVecBlocks = 512;
+constexpr int kStencilRows = 16;  // rows per block of the FP64 stencil kernels (64 x 16 threads)
 constexpr int kFinThreads = 256;
 
 // Split-layout geometry of one level.
@@ -55,6 +56,7 @@ struct State {
     double inv_dt;  // 1/dt of the current operator (0 = steady)
     double tol;
     double bb, rr, rz, rzo, pAp, alpha, beta, relres;
+    double rzb[2];                 // r.z by iteration parity (multigrid path)
     double chk_rr, chk_bb;         // true-residual check
     double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
     int maxit, iters, done, converged, zero_x, flexible;
@@ -129,6 +131,83 @@ __device__ void sum_partials(const double* __restrict__ part, int nblk, double (
             for (int q = 1; q < kFinThreads / 64; ++q) s = ((KMAX >> k) & 1) ? fmax(s, sh[k][q]) : s + sh[k][q];
             tot[k] = s;
         }
+}
+
+// Grid totals of a previous kernel's nblk partials, in block order, in every
+// thread of the calling block (each block of a consumer kernel computes the
+// same totals, so no separate reduction launch is needed).
+template <int NV>
+__device__ void block_sum_partials(const double* __restrict__ part, int nblk, double (&tot)[NV]) {
+    __shared__ double sh[NV][16];
+    const int nthr = blockDim.x * blockDim.y;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6, nw = (nthr + 63) >> 6;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = tid; b < nblk; b += nthr)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += part[(size_t)b * NV + k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = acc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = sh[k][0];
+        for (int q = 1; q < nw; ++q) s += sh[k][q];
+        tot[k] = s;
+    }
+    __syncthreads();
+}
+
+// Convergence test at the head of the V-cycle, from the partials of the
+// kernel before it: the start residual (CONS_START: r.r, b.b) or the x, r
+// update of an iteration (CONS_ITER: r.r, r.z_old). All blocks reach the same
+// decision; block 0 records it. Returns true when the solve is finished
+// (pcg3d_cpu.cpp:141-176: b = 0, ||r||/||b|| < tol, or maxit reached).
+enum { CONS_NONE = 0, CONS_START = 1, CONS_ITER = 2 };
+__device__ inline bool consume_convergence(const double* __restrict__ part, int nblk, int mode, State* st) {
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+    double tot[3] = {0.0, 0.0, 0.0};
+    if (mode == CONS_START) {
+        block_sum_partials<3>(part, nblk, tot);
+    } else {
+        double t2[2];
+        block_sum_partials<2>(part, nblk, t2);
+        tot[0] = t2[0];
+        tot[1] = t2[1];
+    }
+    const double tol = st->tol;
+    if (mode == CONS_START) {
+        const double bb = tot[1];
+        const double relres = bb == 0.0 ? 0.0 : sqrt(tot[0] / bb);
+        const bool fin = bb == 0.0 || relres < tol;
+        if (lead) {
+            st->bb = bb;
+            st->rr = tot[0];
+            st->rzo = 0.0;
+            st->relres = relres;
+            st->zero_x = bb == 0.0;
+            st->converged = fin;
+            st->done = fin;
+        }
+        return fin;
+    }
+    const double relres = sqrt(tot[0] / st->bb);
+    const bool conv = relres < tol;
+    if (lead) {
+        const int it = st->iters + 1;
+        st->iters = it;
+        st->rr = tot[0];
+        st->rzo = tot[1];
+        st->relres = relres;
+        st->converged = conv;
+        st->done = conv || it >= st->maxit;
+    }
+    return conv;
 }
 
 // ---- column addressing ------------------------------------------------------

@@ -23,7 +23,9 @@ namespace th {
 enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 constexpr int kTailThreads = 512;
 constexpr int kMaxTail = 8;
-constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the tail
+constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
+constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail
+constexpr int kTailLdsThreads = 512;            // 256 VGPRs per lane: no spills for nz <= 16
 
 // Exact solve of column (i, j) with its lateral neighbours fixed
 // (mg3d.h:67-84), in place in x. MODE: see above. xc/gc: the coarse level
@@ -41,14 +43,17 @@ __device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__
         qs = gc.pos(i >> 1, (c.hs ? j - 1 : j) >> 1);
         qn = gc.pos(i >> 1, (c.hn ? j + 1 : j) >> 1);
     }
-    T xs[NZ], fcp[NZ];
-    T prev = 0, gd = 0;
+    // Pass 1: every plane's right-hand side with the lateral neighbours held
+    // fixed. No plane depends on another, so all loads are in flight at once,
+    // which is what hides memory latency on the small coarse levels, where a
+    // SIMD holds one or two waves.
+    T s[NZ], inv[NZ], gu[NZ];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) {
         if (l < g.nz) {
             const long base = (long)l * g.plane;
             const long id = base + c.c0;
-            T s = b[id];
+            T v = b[id];
             if (MODE != SW_ZERO) {
                 T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
                 if (MODE == SW_PROLONG) {
@@ -64,23 +69,30 @@ __device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__
                 if (c.he) ls += Ge * xe;
                 if (c.hs) ls += Gs * xsv;
                 if (c.hn) ls += Gn * xn;
-                s = s + ls;
+                v = v + ls;
             }
-            const T lower = (l > 0) ? -gd : T(0);
-            const T inv = finv[id];
-            prev = (s - lower * prev) * inv;
-            xs[l] = prev;
-            const T gu = gU[id];
-            fcp[l] = (l < g.nz - 1) ? (-gu) * inv : T(0);
-            gd = gu;
+            s[l] = v;
+            inv[l] = finv[id];
+            gu[l] = gU[id];
+        }
+    }
+    // Pass 2: Thomas forward elimination and back substitution in registers,
+    // upper factor rebuilt as (-gU) * (1/pivot).
+    T prev = 0;
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < g.nz) {
+            const T lower = (l > 0) ? -gu[l - 1] : T(0);
+            prev = (s[l] - lower * prev) * inv[l];
+            s[l] = prev;
         }
     }
 #pragma unroll
     for (int l = NZ - 2; l >= 0; --l)
-        if (l < g.nz - 1) xs[l] = xs[l] - fcp[l] * xs[l + 1];
+        if (l < g.nz - 1) s[l] = s[l] - ((-gu[l]) * inv[l]) * s[l + 1];
 #pragma unroll
     for (int l = 0; l < NZ; ++l)
-        if (l < g.nz) x[(long)l * g.plane + c.c0] = xs[l];
+        if (l < g.nz) x[(long)l * g.plane + c.c0] = s[l];
 }
 
 // Coarse b at (I, J, l): sum over the (up to 4) children, row-major, of the
@@ -106,11 +118,58 @@ __device__ __forceinline__ void restrict_cell(const Geo& f, const Ops<T>& o, con
     cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
 }
 
+// Coarse b at (I, J, l) from the red lateral sums. After the red sweep on
+// x = 0 and the black sweep, every black column satisfies its equation
+// exactly and a red column's residual is just its lateral term sum G x
+// (the black neighbours it was solved without). So the restricted residual is
+// the sum over the red children (2I, 2J) and (2I+1, 2J+1) of sum_lat G x:
+// equal to restrict_cell in exact arithmetic, with 11 instead of 21 bytes
+// per fine cell (x of one colour, gE, gN; no b, gU, cap).
+template <class T>
+__device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __restrict__ gE, const T* __restrict__ gN,
+                                                   const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb, int I,
+                                                   int J, int l) {
+    const long base = (long)l * f.plane;
+    const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
+    T r[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const Col c = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
+        const T Gw = gE[base + c.pw], Ge = gE[base + c.c0], Gs = gN[base + c.ps], Gn = gN[base + c.c0];
+        const T xw = fx[base + c.pw], xe = fx[base + c.pe], xs = fx[base + c.ps], xn = fx[base + c.pn];
+        T s = 0;
+        if (c.hw) s += Gw * xw;
+        if (c.he) s += Ge * xe;
+        if (c.hs) s += Gs * xs;
+        if (c.hn) s += Gn * xn;
+        r[k] = s;
+    }
+    T sum = 0;
+    sum += r[0];
+    if (two) sum += r[1];
+    cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict_fast(Geo f, Ops<T> o, const T* __restrict__ fx, Geo cg,
+                                                        T* __restrict__ cb, const State* __restrict__ st) {
+    if (st->done) return;
+    const int I = blockIdx.x * kColsX + threadIdx.x;
+    const int J = blockIdx.y * kColsY + threadIdx.y;
+    if (I >= cg.nx || J >= cg.ny) return;
+    restrict_cell_fast<T>(f, o.gE, o.gN, fx, cg, cb, I, J, blockIdx.z);
+}
+
+// One colour sweep of a level. The first sweep of a V-cycle can also carry
+// the convergence test of the kernel before it (cons_mode, see
+// consume_convergence): a converged solve ends here, with no separate launch.
 template <class T, int NZ, int MODE>
 __global__ __launch_bounds__(256) void k_sweep(Geo g, Ops<T> o, const T* __restrict__ finv,
                                                 const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc,
-                                                const T* __restrict__ xc, const State* __restrict__ st) {
+                                                const T* __restrict__ xc, State* st, const double* __restrict__ cons_part,
+                                                int cons_nblk, int cons_mode) {
     if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
     const int t = blockIdx.x * kColsX + threadIdx.x;
     const int j = blockIdx.y * kColsY + threadIdx.y;
     if (j >= g.ny) return;
@@ -175,10 +234,10 @@ struct Tail {
     TailLevel<T> L[kMaxTail];
 };
 
-template <class T, int NZ, int MODE>
+template <class T, int NZ, int MODE, int TT>
 __device__ __forceinline__ void tail_sweep(const TailLevel<T>& L, int colour, const TailLevel<T>& C) {
     const int cols = L.g.hx * L.g.ny;
-    for (int q = threadIdx.x; q < cols; q += kTailThreads) {
+    for (int q = threadIdx.x; q < cols; q += TT) {
         const int t = q % L.g.hx, j = q / L.g.hx;
         const int i = 2 * t + ((j + colour) & 1);
         if (i < L.g.nx)
@@ -187,28 +246,94 @@ __device__ __forceinline__ void tail_sweep(const TailLevel<T>& L, int colour, co
     __syncthreads();
 }
 
-// vcycle(k) for the levels of the tail (pcg3d_cpu.cpp:69-113); its first
-// level's b has been restricted by the caller.
+// Threads of the tail workgroup: 1024 where the column registers fit in
+// 128 VGPRs (FP32, nz <= 32), else 512.
 template <class T, int NZ>
-__global__ __launch_bounds__(kTailThreads) void k_tail(Tail<T> t, const State* __restrict__ st) {
-    if (st->done) return;
-    const T inv_dt = (T)st->inv_dt;
-    for (int k = 0; k + 1 < t.n; ++k) {
-        const TailLevel<T>&L = t.L[k], &C = t.L[k + 1];
-        tail_sweep<T, NZ, SW_ZERO>(L, 0, C);
-        tail_sweep<T, NZ, SW_PLAIN>(L, 1, C);
+constexpr int tail_threads() {
+    return (sizeof(T) == 4 && NZ <= 32) ? 1024 : 512;
+}
+
+// vcycle(k) for the levels of the tail (pcg3d_cpu.cpp:69-113); its first
+// level's b has been restricted by the caller (or, for a grid of at most
+// 32 x 32 columns, written by the PCG kernel whose convergence test the tail
+// then carries: cons_mode).
+// The V-cycle body over levels L[0..n-1] of a tail (global or LDS copies).
+template <class T, int NZ, int TT>
+__device__ void tail_vcycle(const TailLevel<T>* L, int n, T inv_dt, int fast) {
+    for (int k = 0; k + 1 < n; ++k) {
+        const TailLevel<T>&F = L[k], &C = L[k + 1];
+        tail_sweep<T, NZ, SW_ZERO, TT>(F, 0, C);
+        tail_sweep<T, NZ, SW_PLAIN, TT>(F, 1, C);
         const int cells = C.g.nx * C.g.ny * C.g.nz;
-        for (int q = threadIdx.x; q < cells; q += kTailThreads) {
+        for (int q = threadIdx.x; q < cells; q += TT) {
             const int I = q % C.g.nx, rest = q / C.g.nx;
-            restrict_cell<T>(L.g, L.o, L.b, L.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny, inv_dt);
+            if (fast)
+                restrict_cell_fast<T>(F.g, F.o.gE, F.o.gN, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny);
+            else
+                restrict_cell<T>(F.g, F.o, F.b, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny, inv_dt);
         }
         __syncthreads();
     }
-    tail_sweep<T, NZ, SW_ZERO>(t.L[t.n - 1], 0, t.L[t.n - 1]);
-    for (int k = t.n - 2; k >= 0; --k) {
-        tail_sweep<T, NZ, SW_PROLONG>(t.L[k], 1, t.L[k + 1]);
-        tail_sweep<T, NZ, SW_PLAIN>(t.L[k], 0, t.L[k + 1]);
+    tail_sweep<T, NZ, SW_ZERO, TT>(L[n - 1], 0, L[n - 1]);
+    for (int k = n - 2; k >= 0; --k) {
+        tail_sweep<T, NZ, SW_PROLONG, TT>(L[k], 1, L[k + 1]);
+        tail_sweep<T, NZ, SW_PLAIN, TT>(L[k], 0, L[k + 1]);
     }
+}
+
+template <class T, int NZ>
+__global__ __launch_bounds__((tail_threads<T, NZ>())) void k_tail(Tail<T> t, State* st,
+                                                                 const double* __restrict__ cons_part,
+                                                                 int cons_nblk, int cons_mode, int fast) {
+    constexpr int TT = tail_threads<T, NZ>();
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
+    tail_vcycle<T, NZ, TT>(t.L, t.n, (T)st->inv_dt, fast);
+}
+
+// LDS bytes of an LDS-resident tail over levels g[0..n-1]: per cell gE, gN,
+// gU, 1/pivot, x, b (the fast restriction reads no gB and no capacity).
+inline size_t tail_lds_bytes(const Geo* g, int n) {
+    size_t b = 0;
+    for (int k = 0; k < n; ++k) b += 6 * sizeof(float) * (size_t)g[k].cells();
+    return b;
+}
+
+// The same tail with every level copied into LDS first (FP32, fast
+// restriction): the 20-odd barrier-separated phases then wait on LDS
+// (~100-cycle) instead of L2 latency. Only the top level's x goes back to
+// global memory, for the prolongation into the level above.
+template <int NZ>
+__global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, State* st, const double* __restrict__ cons_part,
+                                                   int cons_nblk, int cons_mode) {
+    extern __shared__ float sm[];
+    __shared__ TailLevel<float> S[kMaxTail];  // level views into LDS (not in scratch)
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
+    size_t off = 0;
+    for (int k = 0; k < t.n; ++k) {
+        const TailLevel<float>& G = t.L[k];
+        const long cells = G.g.cells();
+        float* gE = sm + off;
+        float* gN = gE + cells;
+        float* gU = gN + cells;
+        float* fi = gU + cells;
+        float* x = fi + cells;
+        float* b = x + cells;
+        off += 6 * (size_t)cells;
+        for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
+            gE[q] = G.o.gE[q];
+            gN[q] = G.o.gN[q];
+            gU[q] = G.o.gU[q];
+            fi[q] = G.finv[q];
+            if (k == 0) b[q] = G.b[q];
+        }
+        if (threadIdx.x == 0) S[k] = TailLevel<float>{G.g, Ops<float>{gE, gN, gU, nullptr, nullptr}, fi, x, b};
+    }
+    __syncthreads();
+    tail_vcycle<float, NZ, kTailLdsThreads>(S, t.n, 0.0f, 1);
+    const long cells0 = t.L[0].g.cells();
+    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = S[0].x[q];
 }
 
 }  // namespace th

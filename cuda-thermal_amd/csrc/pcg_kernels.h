@@ -3,8 +3,10 @@
 //
 // Stencil kernels run one thread per column (both colours of a row slot) and
 // walk the planes with the vertical neighbours in registers. Every reduction
-// writes one partial per block; a k_finalize launch adds them in block order
-// and runs the epilogue (alpha, beta, convergence test) on the device.
+// writes one partial per block. On the multigrid path the next kernel adds
+// them (every block, in block order) and forms its scalar itself: alpha in
+// k_update, the convergence test at the head of the V-cycle, beta in
+// k_pupdate. The Jacobi path and the checks use a k_finalize launch.
 #pragma once
 #include "kernels.h"
 
@@ -30,14 +32,14 @@ enum {
 // r.r, b.b (and r.z). RES_CHECK: the true residual ||P - A x|| of the steady
 // operator, no writes.
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
+__global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
                                                    const double* __restrict__ x, double* __restrict__ r,
                                                    T* __restrict__ b0, const double* __restrict__ invd,
                                                    double* __restrict__ z, double* __restrict__ p,
                                                    const State* __restrict__ st, double* __restrict__ part) {
     if (MODE != RES_CHECK && st->done) return;
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     double v[3] = {0.0, 0.0, 0.0};  // rr, bb, rz
     const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
     int i;
@@ -66,12 +68,12 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, Ops<T> o, const double*
 
 // Ap = (A + C/dt) p; partial p.Ap.
 template <class T, int NZ>
-__global__ __launch_bounds__(256) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
+__global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
                                                 double* __restrict__ Ap, const State* __restrict__ st,
                                                 double* __restrict__ part) {
     if (st->done) return;
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     double v[1] = {0.0};
     const double inv_dt = st->inv_dt;
     const T* __restrict__ gE = o.gE;
@@ -120,14 +122,28 @@ __global__ __launch_bounds__(256) void k_apply(Geo g, Ops<T> o, const double* __
 // x += alpha p, r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or
 // z = r / diag (Jacobi); partials r.r and r.z_old (flexible beta: z_old is the
 // previous V-cycle output, still in x0) or r.z (Jacobi).
+// Multigrid path: alpha = rz / p.Ap is formed in every block from the apply
+// kernel's partials (apart), so no reduction launch sits between the two.
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_update(long n, const double* __restrict__ p, const double* __restrict__ Ap,
-                                                 double* __restrict__ x, double* __restrict__ r, T* __restrict__ b0,
-                                                 const T* __restrict__ zold, const double* __restrict__ invd,
-                                                 double* __restrict__ z, const State* __restrict__ st,
-                                                 double* __restrict__ part) {
+__global__ __launch_bounds__(kVecThreads) void k_update(long n, const double* __restrict__ p,
+                                                         const double* __restrict__ Ap, double* __restrict__ x,
+                                                         double* __restrict__ r, T* __restrict__ b0,
+                                                         const T* __restrict__ zold, const double* __restrict__ invd,
+                                                         double* __restrict__ z, State* st, double* __restrict__ part,
+                                                         const double* __restrict__ apart, int anblk) {
     if (st->done) return;
-    const double alpha = st->alpha;
+    double alpha;
+    if (apart) {
+        double pAp[1];
+        block_sum_partials<1>(apart, anblk, pAp);
+        alpha = st->rzb[st->iters & 1] / pAp[0];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->pAp = pAp[0];
+            st->alpha = alpha;
+        }
+    } else {
+        alpha = st->alpha;
+    }
     double v[2] = {0.0, 0.0};
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -149,8 +165,8 @@ __global__ __launch_bounds__(256) void k_update(long n, const double* __restrict
 
 // Partial r.z with z = the V-cycle output.
 template <class T>
-__global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
-                                              const State* __restrict__ st, double* __restrict__ part) {
+__global__ __launch_bounds__(kVecThreads) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
+                                                      const State* __restrict__ st, double* __restrict__ part) {
     if (st->done) return;
     double v[1] = {0.0};
     const long stride = (long)gridDim.x * blockDim.x;
@@ -158,12 +174,31 @@ __global__ __launch_bounds__(256) void k_dot(long n, const double* __restrict__ 
     block_partials<1>(v, part);
 }
 
-// p = z + beta p (first: p = z).
+// p = z + beta p (first: p = z). Multigrid path: every block forms r.z from
+// the dot kernel's partials (dpart) and beta from it - Fletcher-Reeves
+// rz_new / rz_old or flexible (rz_new - r.z_old) / rz_old - with rz kept by
+// iteration parity, so blocks never read a value another block rewrites.
 template <class Z, int FIRST>
-__global__ __launch_bounds__(256) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
-                                                  const State* __restrict__ st) {
+__global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
+                                                          State* st, const double* __restrict__ dpart, int dnblk) {
     if (st->done) return;
-    const double beta = st->beta;
+    double beta = 0.0;
+    if (dpart) {
+        double rz[1];
+        block_sum_partials<1>(dpart, dnblk, rz);
+        const int it = st->iters;
+        if (!FIRST) {
+            const double old = st->rzb[(it + 1) & 1];
+            beta = st->flexible ? (rz[0] - st->rzo) / old : rz[0] / old;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rzb[it & 1] = rz[0];
+            st->rz = rz[0];
+            st->beta = beta;
+        }
+    } else {
+        beta = st->beta;
+    }
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
         p[k] = FIRST ? (double)z[k] : (double)z[k] + beta * p[k];
@@ -173,9 +208,12 @@ __global__ __launch_bounds__(256) void k_pupdate(long n, const Z* __restrict__ z
 template <int NV, int EPI, unsigned KMAX = 0>
 __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restrict__ part, int nblk, State* st) {
     if (EPI != FIN_CHECK && EPI != FIN_ENERGY && st->done) return;
-    double tot[NV];
-    sum_partials<NV, KMAX>(part, nblk, tot);
+    double tn[NV];
+    sum_partials<NV, KMAX>(part, nblk, tn);
     if (threadIdx.x != 0) return;
+    double tot[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = tn[k];
     if (EPI == FIN_RES || EPI == FIN_RES_JACOBI) {  // pcg3d_cpu.cpp:141-149
         st->bb = tot[1];
         st->rr = tot[0];
@@ -276,7 +314,7 @@ __global__ __launch_bounds__(256) void k_invdiag(Geo g, Ops<T> o, double* __rest
 
 // b = P + C/dt x (pcg3d_gpu.cu:104-109).
 template <class T>
-__global__ __launch_bounds__(256) void k_rhs_transient(long n, const T* __restrict__ cap, const double* __restrict__ P,
+__global__ __launch_bounds__(kVecThreads) void k_rhs_transient(long n, const T* __restrict__ cap, const double* __restrict__ P,
                                                         const double* __restrict__ x, double* __restrict__ b,
                                                         const State* __restrict__ st) {
     const double inv_dt = st->inv_dt;
@@ -288,7 +326,7 @@ __global__ __launch_bounds__(256) void k_rhs_transient(long n, const T* __restri
 // Energy terms of the current x: sum C x (heat content), heat out through the
 // top and bottom (stack3d.h:254-263), and max x over cells (not padding).
 template <class T>
-__global__ __launch_bounds__(256) void k_energy(Geo g, Ops<T> o, const double* __restrict__ x,
+__global__ __launch_bounds__(kVecThreads) void k_energy(Geo g, Ops<T> o, const double* __restrict__ x,
                                                  double* __restrict__ part) {
     double v[3] = {0.0, 0.0, -1.0e300};
     const long n = g.cells(), top = (long)(g.nz - 1) * g.plane;

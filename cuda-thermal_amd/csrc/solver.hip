@@ -54,6 +54,10 @@ static unsigned vec_blocks(long n) {
     return (unsigned)std::max(1L, std::min((long)kMaxVecBlocks, (n + kVecThreads - 1) / kVecThreads));
 }
 static unsigned lin_blocks(long n) { return (unsigned)((n + 255) / 256); }
+static dim3 stencil_block() { return dim3(kColsX, kStencilRows); }
+static dim3 stencil_grid(int slots, int rows) {
+    return dim3((unsigned)((slots + kColsX - 1) / kColsX), (unsigned)((rows + kStencilRows - 1) / kStencilRows));
+}
 
 template <class T>
 struct DevLevel {
@@ -193,12 +197,38 @@ struct Solver final : SolverBase {
             if (!mg()) break;
         }
         if ((rc = dalloc(&part, maxblk * 4))) return rc;
-        // Levels with at most kTailCols columns (and at most kMaxTail of
-        // them) run in the single-workgroup tail kernel.
+        {
+            const dim3 sg = stencil_grid(2 * g0.hx, g0.ny);
+            nblk_stencil = sg.x * sg.y;
+            nblk_vec = vec_blocks(n0);
+            if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_stencil)) ||
+                (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
+                return rc;
+        }
+        // The lowest levels run in one workgroup. FP32 with the lateral
+        // restriction and nz <= 16: as many levels as fit in LDS (k_tail_lds).
+        // Otherwise: levels with at most kTailCols columns (k_tail, L2).
+        fast = opt.restriction == TH_RESTRICT_LATERAL;
         tail_k0 = L.size() - 1;
-        while (tail_k0 > 0 && (long)L[tail_k0 - 1].g.nx * L[tail_k0 - 1].g.ny <= kTailCols &&
-               L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
-            --tail_k0;
+        lds_tail = fast && sizeof(T) == 4 && g0.nz <= 16;
+        if (lds_tail) {
+            auto bytes_from = [&](size_t k0) {
+                std::vector<Geo> gs;
+                for (size_t k = k0; k < L.size(); ++k) gs.push_back(L[k].g);
+                return tail_lds_bytes(gs.data(), (int)gs.size());
+            };
+            while (tail_k0 > 0 && bytes_from(tail_k0 - 1) <= kTailLdsBytes && L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
+                --tail_k0;
+            lds_bytes = bytes_from(tail_k0);
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<8>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<12>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<16>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+        } else {
+            while (tail_k0 > 0 && (long)L[tail_k0 - 1].g.nx * L[tail_k0 - 1].g.ny <= kTailCols &&
+                   L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
+                --tail_k0;
+        }
         if ((rc = upload_split<double, double>(g0, g0.nz, m.P, P))) return rc;
         if ((rc = set_dt(0.0))) return rc;
         HIPTRY(hipStreamSynchronize(s));
@@ -279,21 +309,39 @@ struct Solver final : SolverBase {
 
     // ---- launch sequences -------------------------------------------------
 
-    size_t tail_k0 = 0;  // first level handled by the single-workgroup tail
+    size_t tail_k0 = 0;     // first level handled by the single-workgroup tail
+    bool fast = true;       // lateral (fast) restriction
+    bool lds_tail = false;  // tail levels held in LDS
+    size_t lds_bytes = 0;
+
+    // Partial buffers, one per producing kernel, so a consumer can still read
+    // its producer's partials while the next reduction runs.
+    double *part_res = nullptr, *part_apply = nullptr, *part_upd = nullptr, *part_dot = nullptr;
+    unsigned nblk_stencil = 0, nblk_vec = 0;
 
     template <int NV, int EPI, unsigned KMAX = 0>
-    int finalize(unsigned nblk) {
-        k_finalize<NV, EPI, KMAX><<<1, kFinThreads, 0, s>>>(part, (int)nblk, st);
+    int finalize(unsigned nblk, const double* pp = nullptr) {
+        k_finalize<NV, EPI, KMAX><<<1, kFinThreads, 0, s>>>(pp ? pp : part, (int)nblk, st);
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
 
+    // Consumer of the previous kernel's partials, carried by the first kernel
+    // of the V-cycle (see consume_convergence).
+    struct Cons {
+        const double* part;
+        int nblk;
+        int mode;
+    };
+
     template <int MODE>
-    int launch_sweep(size_t k, int colour) {
+    int launch_sweep(size_t k, int colour, Cons cs = {nullptr, 0, CONS_NONE}) {
         const DevLevel<T>& d = L[k];
         const DevLevel<T>& c = L[std::min(k + 1, L.size() - 1)];
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
-#define TH_SWEEP(NZ) k_sweep<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, d.b, d.x, colour, c.g, c.x, st)
+#define TH_SWEEP(NZ)                                                                                         \
+    k_sweep<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, d.b, d.x, colour, c.g, c.x, st, \
+                                                      cs.part, cs.nblk, cs.mode)
         if (d.g.nz <= 4) TH_SWEEP(4);
         else if (d.g.nz <= 8) TH_SWEEP(8);
         else if (d.g.nz <= 12) TH_SWEEP(12);
@@ -307,7 +355,7 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 
-    int launch_tail() {
+    int launch_tail(Cons cs) {
         Tail<T> t{};
         t.n = (int)(L.size() - tail_k0);
         for (int k = 0; k < t.n; ++k) {
@@ -315,7 +363,19 @@ struct Solver final : SolverBase {
             t.L[k] = TailLevel<T>{d.g, d.ops(), d.finv, d.x, d.b};
         }
         const int nz = g0.nz;
-#define TH_TAIL(NZ) k_tail<T, NZ><<<1, kTailThreads, 0, s>>>(t, st)
+        if (lds_tail) {
+            const Tail<float>& tf = *reinterpret_cast<const Tail<float>*>(&t);
+#define TH_TAIL(NZ) k_tail_lds<NZ><<<1, kTailLdsThreads, lds_bytes, s>>>(tf, st, cs.part, cs.nblk, cs.mode)
+            if (nz <= 4) TH_TAIL(4);
+            else if (nz <= 8) TH_TAIL(8);
+            else if (nz <= 12) TH_TAIL(12);
+            else TH_TAIL(16);
+#undef TH_TAIL
+            HIPTRY(hipGetLastError());
+            return TH_OK;
+        }
+#define TH_TAIL(NZ) \
+    k_tail<T, NZ><<<1, tail_threads<T, NZ>(), 0, s>>>(t, st, cs.part, cs.nblk, cs.mode, fast ? 1 : 0)
         if (nz <= 4) TH_TAIL(4);
         else if (nz <= 8) TH_TAIL(8);
         else if (nz <= 12) TH_TAIL(12);
@@ -329,30 +389,40 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 
-    // One V-cycle on level k (pcg3d_cpu.cpp:69-113): pre-smooth red, black;
-    // restrict; coarse V-cycle; post-smooth black (with the prolongation),
-    // red. Level k's x starts from zero. Levels from tail_k0 down run in the
-    // single-workgroup tail kernel.
-    int vcycle(size_t k) {
-        int rc;
-        if (k == tail_k0) return launch_tail();
-        if ((rc = launch_sweep<SW_ZERO>(k, 0))) return rc;
-        if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
+    int launch_restrict(size_t k) {
         const DevLevel<T>& f = L[k];
         const DevLevel<T>& c = L[k + 1];
         dim3 grid = col_grid(c.g.nx, c.g.ny);
         grid.z = c.g.nz;
-        k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
+        if (fast)
+            k_restrict_fast<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.x, c.g, c.b, st);
+        else
+            k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
         HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
+    // One V-cycle on level k (pcg3d_cpu.cpp:69-113): pre-smooth red, black;
+    // restrict; coarse V-cycle; post-smooth black (with the prolongation),
+    // red. Level k's x starts from zero. Levels from tail_k0 down run in the
+    // single-workgroup tail kernel. `cs`: convergence test carried by the
+    // V-cycle's first kernel (level 0 only).
+    int vcycle(size_t k, Cons cs = {nullptr, 0, CONS_NONE}) {
+        int rc;
+        if (k == tail_k0) return launch_tail(cs);
+        if ((rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
+        if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
+        if ((rc = launch_restrict(k))) return rc;
         if ((rc = vcycle(k + 1))) return rc;
         if ((rc = launch_sweep<SW_PROLONG>(k, 1))) return rc;
         return launch_sweep<SW_PLAIN>(k, 0);
     }
 
-    int launch_apply() {
-        const dim3 grid = col_grid(2 * g0.hx, g0.ny);
+    // Ap = (A + C/dt) p with partials p.Ap in part_apply.
+    int launch_apply_kernel() {
+        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const Ops<T> o = L[0].ops();
-#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, p, Ap, st, part)
+#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, stencil_block(), 0, s>>>(g0, o, p, Ap, st, part_apply)
         if (g0.nz <= 4) TH_APPLY(4);
         else if (g0.nz <= 8) TH_APPLY(8);
         else if (g0.nz <= 12) TH_APPLY(12);
@@ -363,62 +433,64 @@ struct Solver final : SolverBase {
         else TH_APPLY(64);
 #undef TH_APPLY
         HIPTRY(hipGetLastError());
-        return finalize<1, FIN_ALPHA>(grid.x * grid.y);
+        return TH_OK;
+    }
+    int launch_apply() {  // with alpha in the state (Jacobi path and tests)
+        int rc;
+        if ((rc = launch_apply_kernel())) return rc;
+        return finalize<1, FIN_ALPHA>(nblk_stencil, part_apply);
     }
 
     // Start: residual, first preconditioner application, rz, p = z.
     int enqueue_start() {
-        const dim3 grid = col_grid(2 * g0.hx, g0.ny);
+        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const unsigned vb = vec_blocks(n0);
         int rc;
         if (mg()) {
-            k_residual<T, RES_MG><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
-                                                               nullptr, st, part);
+            k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
+                                                                   nullptr, st, part_res);
             HIPTRY(hipGetLastError());
-            if ((rc = finalize<3, FIN_RES>(grid.x * grid.y))) return rc;
-            if ((rc = vcycle(0))) return rc;
-            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
+            if ((rc = vcycle(0, Cons{part_res, (int)nblk_stencil, CONS_START}))) return rc;
+            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
-            if ((rc = finalize<1, FIN_DOT_FIRST>(vb))) return rc;
-            k_pupdate<T, 1><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st);
+            k_pupdate<T, 1><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
             HIPTRY(hipGetLastError());
             return TH_OK;
         }
-        k_residual<T, RES_JACOBI><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
-                                                               part);
+        k_residual<T, RES_JACOBI><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
+                                                                   part_res);
         HIPTRY(hipGetLastError());
-        return finalize<3, FIN_RES_JACOBI>(grid.x * grid.y);
+        return finalize<3, FIN_RES_JACOBI>(nblk_stencil, part_res);
     }
 
-    // One PCG iteration: Ap and alpha; x, r update and the convergence test;
-    // z = M^-1 r; beta; p = z + beta p.
+    // One PCG iteration: Ap; x, r update (alpha formed from the apply
+    // partials); convergence test + z = M^-1 r (V-cycle); r.z; p = z + beta p
+    // (beta formed from the dot partials). Multigrid: 3 + V-cycle launches.
     int enqueue_iter() {
         const unsigned vb = vec_blocks(n0);
         int rc;
-        if ((rc = launch_apply())) return rc;
         if (mg()) {
-            if (opt.beta == TH_BETA_FLEXIBLE) {
+            if ((rc = launch_apply_kernel())) return rc;
+            if (opt.beta == TH_BETA_FLEXIBLE)
                 k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, L[0].x, nullptr, nullptr,
-                                                                   st, part);
-                HIPTRY(hipGetLastError());
-                if ((rc = finalize<2, FIN_UPDATE_FLEX>(vb))) return rc;
-            } else {
+                                                                   st, part_upd, part_apply, (int)nblk_stencil);
+            else
                 k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, nullptr, nullptr, nullptr, st,
-                                                              part);
-                HIPTRY(hipGetLastError());
-                if ((rc = finalize<2, FIN_UPDATE>(vb))) return rc;
-            }
-            if ((rc = vcycle(0))) return rc;
-            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part);
+                                                              part_upd, part_apply, (int)nblk_stencil);
             HIPTRY(hipGetLastError());
-            if ((rc = finalize<1, FIN_DOT>(vb))) return rc;
-            k_pupdate<T, 0><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st);
-        } else {
-            k_update<T, UP_JACOBI><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, nullptr, nullptr, invd, z, st, part);
+            if ((rc = vcycle(0, Cons{part_upd, (int)vb, CONS_ITER}))) return rc;
+            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
-            if ((rc = finalize<2, FIN_UPDATE_JACOBI>(vb))) return rc;
-            k_pupdate<double, 0><<<vb, kVecThreads, 0, s>>>(n0, z, p, st);
+            k_pupdate<T, 0><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
+            HIPTRY(hipGetLastError());
+            return TH_OK;
         }
+        if ((rc = launch_apply())) return rc;
+        k_update<T, UP_JACOBI><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, nullptr, nullptr, invd, z, st, part_upd,
+                                                          nullptr, 0);
+        HIPTRY(hipGetLastError());
+        if ((rc = finalize<2, FIN_UPDATE_JACOBI>(vb, part_upd))) return rc;
+        k_pupdate<double, 0><<<vb, kVecThreads, 0, s>>>(n0, z, p, st, nullptr, 0);
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
@@ -498,9 +570,9 @@ struct Solver final : SolverBase {
     }
 
     int true_relres(double* out) override {
-        const dim3 grid = col_grid(2 * g0.hx, g0.ny);
-        k_residual<T, RES_CHECK><<<grid, col_block(), 0, s>>>(g0, L[0].ops(), P, x, nullptr, nullptr, nullptr,
-                                                              nullptr, nullptr, st, part);
+        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
+        k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, x, nullptr, nullptr, nullptr,
+                                                                  nullptr, nullptr, st, part);
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = finalize<3, FIN_CHECK>(grid.x * grid.y)) || (rc = fetch_state())) return rc;
@@ -559,26 +631,10 @@ struct Solver final : SolverBase {
                 case TH_K_SWEEP:
                     return launch_sweep<SW_PLAIN>(0, 0);
                 case TH_K_APPLY: {
-                    const dim3 grid = col_grid(2 * g0.hx, g0.ny);
-                    const Ops<T> o = L[0].ops();
-#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, p, Ap, st, part)
-                    if (g0.nz <= 8) TH_APPLY(8);
-                    else if (g0.nz <= 12) TH_APPLY(12);
-                    else if (g0.nz <= 16) TH_APPLY(16);
-                    else if (g0.nz <= 32) TH_APPLY(32);
-                    else TH_APPLY(64);
-#undef TH_APPLY
-                    HIPTRY(hipGetLastError());
-                    return TH_OK;
+                    return launch_apply_kernel();
                 }
-                case TH_K_RESTRICT: {
-                    const DevLevel<T>&f = L[0], &c = L[1];
-                    dim3 grid = col_grid(c.g.nx, c.g.ny);
-                    grid.z = c.g.nz;
-                    k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
-                    HIPTRY(hipGetLastError());
-                    return TH_OK;
-                }
+                case TH_K_RESTRICT:
+                    return launch_restrict(0);
                 case TH_K_COPY:
                     k_cast<double, double><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, p, Ap);
                     HIPTRY(hipGetLastError());
@@ -592,7 +648,9 @@ struct Solver final : SolverBase {
         switch (which) {
             case TH_K_SWEEP: *bytes = 4.5 * sz * N; break;                  // b, x(other), gE, gN, gU, 1/pivot; x
             case TH_K_APPLY: *bytes = (16.0 + 3.0 * sz) * N; break;         // p, Ap; gE, gN, gU
-            case TH_K_RESTRICT: *bytes = 5.0 * sz * N + 0.25 * sz * N; break;  // b, x, gE, gN, gU; coarse b
+            case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN; residual form: b, x, gE, gN, gU; coarse b
+                *bytes = (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
+                break;
             case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
             default: *bytes = 0.0;
         }
@@ -647,6 +705,7 @@ void th_solver_default_opts(th_solver_opts* o) {
     o->use_graphs = 1;
     o->device = 0;
     o->true_residual = 0;
+    o->restriction = TH_RESTRICT_LATERAL;
 }
 
 static int check_desc(const th_model_desc* m) {

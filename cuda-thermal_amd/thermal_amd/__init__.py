@@ -23,6 +23,7 @@ ERRORS = {1: "invalid argument", 2: "HIP error", 3: "model not finalized", 4: "c
 PRECOND = {"jacobi": 0, "mg": 1, "multigrid": 1}
 PRECISION = {"mixed": 0, "fp64": 1}
 BETA = {"fr": 0, "fletcher-reeves": 0, "flexible": 1}
+RESTRICTION = {"lateral": 0, "residual": 1}
 
 # Layers of the reference's default two-tier stack (stack3d.h:153-162):
 # (name, thickness m, k W/mK, rho_c J/m^3K, planes, active)
@@ -49,7 +50,7 @@ class ModelDesc(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("precond", C.c_int), ("precision", C.c_int), ("beta", C.c_int), ("use_graphs", C.c_int),
-                ("device", C.c_int), ("true_residual", C.c_int)]
+                ("device", C.c_int), ("true_residual", C.c_int), ("restriction", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -310,8 +311,10 @@ def desc_from_arrays(a):
     return d, keep
 
 
-def make_opts(precond="mg", precision="mixed", beta="flexible", graphs=True, device=0, true_residual=False):
-    return SolverOpts(PRECOND[precond], PRECISION[precision], BETA[beta], int(graphs), device, int(true_residual))
+def make_opts(precond="mg", precision="mixed", beta="flexible", graphs=True, device=0, true_residual=False,
+              restriction="lateral"):
+    return SolverOpts(PRECOND[precond], PRECISION[precision], BETA[beta], int(graphs), device, int(true_residual),
+                      RESTRICTION[restriction])
 
 
 class Solver:
@@ -319,7 +322,7 @@ class Solver:
     Model or a dict of arrays (gE, gN, gU, gB, cap, P, nx, ny, nz, dx, dy)."""
 
     def __init__(self, model, precond="mg", precision="mixed", beta="flexible", graphs=True, device=0,
-                 true_residual=False):
+                 true_residual=False, restriction="lateral"):
         if isinstance(model, Model):
             d, self._keep = model.desc(), model
             self.nx, self.ny, self.nz = model.nx, model.ny, model.nz
@@ -329,7 +332,7 @@ class Solver:
         self.n = self.nx * self.ny * self.nz
         self.h = None
         h = _vp()
-        opts = make_opts(precond, precision, beta, graphs, device, true_residual)
+        opts = make_opts(precond, precision, beta, graphs, device, true_residual, restriction)
         _check(lib().th_solver_create(C.byref(d), C.byref(opts), C.byref(h)))
         self.h = h
 

@@ -114,7 +114,15 @@ typedef struct {
     int true_residual; /* 1: on convergence recompute b - A x in FP64 and restart
                           from x while it is above tol (up to 4 restarts); 0: stop
                           on the recurrence residual, as the reference does */
+    int restriction;   /* TH_RESTRICT_*: how the V-cycle restricts its residual */
 } th_solver_opts;
+
+/* TH_RESTRICT_LATERAL: coarse b = sum over the red children of their lateral
+ * terms sum G x, the residual left after the red (x = 0) and black sweeps,
+ * equal to the reference's sum of (b - A x) over the children in exact
+ * arithmetic and reading half the bytes (default). TH_RESTRICT_RESIDUAL: the
+ * reference's form (pcg3d_cpu.cpp:87-100), operation for operation. */
+enum { TH_RESTRICT_LATERAL = 0, TH_RESTRICT_RESIDUAL = 1 };
 
 TH_API void th_solver_default_opts(th_solver_opts* o);
 

@@ -402,6 +402,7 @@ struct SolverBase {
     virtual int nlevels() const = 0;
     virtual const LevelOp& level(int k) const = 0;
     virtual void level_factors(int k, double* fcp, double* finv) const = 0;
+    bool fast_restrict = false;  // restriction from the red lateral sums (see vcycle)
 };
 
 template <class MgR>
@@ -487,6 +488,21 @@ struct Solver : SolverBase {
         sweep(1);
         Level& C = levels[k + 1];
         const size_t fxy = (size_t)lnx * lny, cxy = (size_t)C.h.nx * C.h.ny;
+        if (fast_restrict) {
+            // After red (on x = 0) then black, every black column satisfies its
+            // equation exactly and each red column's residual is its lateral
+            // sum: coarse b = sum over the red children (2I, 2J), (2I+1, 2J+1)
+            // of sum_lat G x. Equal to the restriction below in exact
+            // arithmetic; no reference counterpart.
+            for (int l = 0; l < lnz; ++l)
+                for (int J = 0; J < C.h.ny; ++J)
+                    for (int I = 0; I < C.h.nx; ++I) {
+                        MgR sum = 0;
+                        sum += lat_sum<MgR>(o, L.x.data(), 2 * I, 2 * J, l);
+                        if (2 * I + 1 < lnx && 2 * J + 1 < lny) sum += lat_sum<MgR>(o, L.x.data(), 2 * I + 1, 2 * J + 1, l);
+                        C.b[(size_t)l * cxy + (size_t)J * C.h.nx + I] = sum;
+                    }
+        } else
         for (int l = 0; l < lnz; ++l)
             for (int J = 0; J < C.h.ny; ++J)
                 for (int I = 0; I < C.h.nx; ++I) {
@@ -777,6 +793,7 @@ void or_solver_level_get(void* s, int k, int which, float* out) {
     const std::vector<float>* v[] = {&L.gE, &L.gN, &L.gU, &L.gB, &L.cap};
     std::copy(v[which]->begin(), v[which]->end(), out);
 }
+void or_solver_set_fast_restrict(void* s, int on) { ((SolverBase*)s)->fast_restrict = on != 0; }
 void or_solver_level_factors(void* s, int k, double* fcp, double* finv) {
     ((SolverBase*)s)->level_factors(k, fcp, finv);
 }

@@ -91,6 +91,7 @@ def lib():
         L.or_solver_level_dims.argtypes = [_vp, C.c_int, _ip, _ip, _ip]
         L.or_solver_level_get.argtypes = [_vp, C.c_int, C.c_int, _fp]
         L.or_solver_level_factors.argtypes = [_vp, C.c_int, _dp, _dp]
+        L.or_solver_set_fast_restrict.argtypes = [_vp, C.c_int]
         L.or_true_relres.argtypes = [C.c_int] * 3 + [_fp] * 5 + [_dp, _dp]
         L.or_true_relres.restype = C.c_double
         L.or_true_relres_flux.argtypes = [C.c_int] * 3 + [_fp] * 5 + [_dp, _dp]
@@ -214,7 +215,7 @@ class Solver:
     """Oracle CpuPcg (pcg3d_cpu.cpp restated). mg_double selects the V-cycle
     precision (False: float, the reference default MgReal)."""
 
-    def __init__(self, arrays, precond="mg", mg_double=False, flexible=False):
+    def __init__(self, arrays, precond="mg", mg_double=False, flexible=False, fast_restrict=False):
         a = arrays
         self.nx, self.ny, self.nz = a["nx"], a["ny"], a["nz"]
         self.n = self.nx * self.ny * self.nz
@@ -223,6 +224,7 @@ class Solver:
         self.h = lib().or_solver_new(self.nx, self.ny, self.nz, a["dx"], a["dy"],
                                      *[_f(v) for v in self._keep], _d(self._P), PRECOND[precond],
                                      int(mg_double), int(flexible))
+        lib().or_solver_set_fast_restrict(self.h, int(fast_restrict))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -290,8 +292,8 @@ class Solver:
         return out
 
 
-def steady(arrays, tol=1e-10, maxit=50000, precond="mg", mg_double=False, flexible=False):
-    s = Solver(arrays, precond, mg_double, flexible)
+def steady(arrays, tol=1e-10, maxit=50000, precond="mg", mg_double=False, flexible=False, fast_restrict=False):
+    s = Solver(arrays, precond, mg_double, flexible, fast_restrict)
     s.set_x_zero()
     s.set_rhs_steady()
     st = s.solve(tol, maxit)

@@ -51,13 +51,18 @@ def test_operator_bitwise(example_odd, precision, dt):
 
 @pytest.mark.parametrize("precision,mg_double", [("mixed", False), ("fp64", True)])
 @pytest.mark.parametrize("dt", [0.0, 1e-3])
-def test_vcycle_bitwise(example_odd, precision, mg_double, dt):
-    a = example_odd.arrays()
-    s = T.Solver(example_odd, precision=precision)
+@pytest.mark.parametrize("restriction", ["lateral", "residual"])
+@pytest.mark.parametrize("n", [(37, 23), (16, 12)])
+def test_vcycle_bitwise(precision, mg_double, dt, restriction, n):
+    """One V-cycle, bit for bit: (37, 23) runs level 0 as separate kernels,
+    (16, 12) runs every level inside the single-workgroup tail."""
+    m = T.Model.example_case(*n)
+    a = m.arrays()
+    s = T.Solver(m, precision=precision, restriction=restriction)
     s.set_dt(dt)
-    o = O.Solver(a, "mg", mg_double=mg_double)
+    o = O.Solver(a, "mg", mg_double=mg_double, fast_restrict=(restriction == "lateral"))
     o.set_dt(dt)
-    r = rng_vec(example_odd.n, 2)
+    r = rng_vec(m.n, 2)
     zg, zo = s.precondition(r), o.precond(r)
     assert np.array_equal(zg, zo), np.abs(zg - zo).max()
 
@@ -72,11 +77,13 @@ def test_jacobi_precond_matches(example_odd):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("beta", ["flexible", "fr"])
-def test_steady_vs_oracle(precision, beta):
+@pytest.mark.parametrize("restriction", ["lateral", "residual"])
+def test_steady_vs_oracle(precision, beta, restriction):
     m = hotspot(64)
     a = m.arrays()
-    ref = O.steady(a, tol=1e-10, mg_double=(precision == "fp64"), flexible=(beta == "flexible"))
-    s = T.Solver(m, precision=precision, beta=beta)
+    ref = O.steady(a, tol=1e-10, mg_double=(precision == "fp64"), flexible=(beta == "flexible"),
+                   fast_restrict=(restriction == "lateral"))
+    s = T.Solver(m, precision=precision, beta=beta, restriction=restriction)
     s.set_x_zero()
     s.set_rhs_steady()
     st = s.solve(1e-10, 200)
@@ -88,6 +95,18 @@ def test_steady_vs_oracle(precision, beta):
     tr = s.true_relres()
     assert tr <= 1.5e-10
     assert abs(tr - O.true_relres(a, x)) <= 1e-3 * tr + 1e-15
+
+
+@pytest.mark.parametrize("n", [(20, 12), (9, 7)])
+def test_steady_small_grid_in_tail(n):
+    """Grids whose every level runs in the single-workgroup tail, which then
+    also carries the convergence test."""
+    m = T.Model.example_case(*n)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-10, flexible=True, fast_restrict=True)
+    r = T.steady_solve(m, tol=1e-10)
+    assert r["converged"] and abs(r["iters"] - ref["iters"]) <= 1
+    assert np.abs(r["T"] - ref["T"]).max() < 1e-8 * ref["T"].max()
 
 
 def test_steady_odd_grid_with_tsvs(example_odd):

@@ -253,7 +253,29 @@ def test_vcycle_is_symmetric():
     assert u @ s.precond(u) > 0
 
 
-# ---- PCG (pcg3d_cpu.cpp) ----------------------------------------------------
+@pytest.mark.parametrize("dt", [0.0, 1e-3])
+def test_lateral_restriction_equals_residual_restriction(dt):
+    """After the red sweep from x = 0 and the black sweep, the residual of
+    every black cell is zero and that of a red cell is its lateral sum G x;
+    restricting the red children's lateral sums therefore gives the
+    reference's restriction of b - A x, which the fixture above pins.  In
+    FP64 the two V-cycles agree to rounding (a dropped child or a wrong sign
+    would differ at O(1))."""
+    a = example(21, 14).arrays()
+    f, s = (O.Solver(a, "mg", mg_double=True, fast_restrict=k) for k in (True, False))
+    f.set_dt(dt)
+    s.set_dt(dt)
+    r = np.random.default_rng(5).standard_normal(21 * 14 * 10)
+    zf, zs = f.precond(r), s.precond(r)
+    assert np.abs(zf - zs).max() <= 1e-12 * np.abs(zs).max()
+    u = np.random.default_rng(6).standard_normal(r.size)
+    assert np.isclose(u @ f.precond(r), r @ f.precond(u), rtol=1e-11)
+    f32 = O.Solver(a, "mg", fast_restrict=True)
+    f32.set_dt(dt)
+    assert np.abs(f32.precond(r) - zs).max() <= 1e-5 * np.abs(zs).max()
+
+
+# ---- PCG (pcg3d_cpu.cpp) ---------------------------------------------------
 
 def test_pcg_matches_reference_fixture():
     f = fx("steady_hotspot_24")
