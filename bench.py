@@ -104,6 +104,30 @@ def cpu_baseline(folder, n, iters):
                 ms_per_iteration=round(per_it, 1))
 
 
+def transient_leg():
+    """The 800x800x10 backward-Euler run from ambient: 100 steps, dt0 = 10 us
+    growing 10% per step with no cap, each step warm-started and replayed
+    as graphs; per-step energy balance and wall time for the 100 steps."""
+    m = T.hotspot_model(800, 5e-3, 5e-3)
+    P = m.total_power()
+    kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = T.Solver(m, precision=prec)
+        T.run_transient(s, **dict(kw, max_steps=3))  # warm-up
+        g = T.run_transient(s, **kw)
+        out[prec] = dict(ms=round(g["solve_ms"], 2), steps=len(g["t"]), t_end_s=float(g["t"][-1]),
+                         iterations=int(g["iters"].sum()), converged=g["converged"],
+                         energy_balance_max=float((np.abs(g["q_out"] + g["q_stored"] - P) / P).max()),
+                         T_max_C=float(g["T_max"][-1]) + 25.0)
+        out[prec + "_T"] = g["T"]
+        s.close()
+        log(f"  transient {prec}: {out[prec]}")
+    out["max_abs_err_mixed_vs_fp64_K"] = float(np.abs(out.pop("mixed_T") - out.pop("fp64_T")).max())
+    out["workload"] = "800x800x10 two-tier stack, 25 W + 25 W with a 3 W hotspot at (5,5) mm, 100 BE steps"
+    return out
+
+
 def kernels_only(n):
     """Sweep and copy kernels only (for counter runs under rocprofv3 --pmc)."""
     folder, m = make_case(n, 1000)
@@ -125,6 +149,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
+    ap.add_argument("--no-transient", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single solve "
@@ -160,6 +185,8 @@ def main():
         with open(cnt) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
+    tr = None if args.no_transient else transient_leg()
+
     cpu = None
     if not args.no_cpu:
         log("  cpu baseline (bounded sample) ...")
@@ -190,6 +217,7 @@ def main():
                      "bytes_per_launch": sweep_bytes, "avg_launch_us": round(sweep_ms * 1e3, 2)},
         "kernels_GBps": {"copy": round(copy_bytes / copy_ms / 1e6, 1), "apply": round(apply_bytes / apply_ms / 1e6, 1),
                          "sweep": round(achieved, 1)},
+        "transient": tr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -52,6 +52,14 @@ inline Geo make_geo(int nx, int ny, int nz) {
 
 // Device-resident PCG scalars and flags (the role of the reference's sc[]
 // slots, pcg3d_gpu.cu:80-131).
+// One backward-Euler step as recorded on the device (layout of
+// th_step_record: StepRecord, src/transient3d.h:29-35).
+struct StepRec {
+    double t, dt;
+    int iters;
+    double T_max, q_out, q_stored;
+};
+
 struct State {
     double inv_dt;  // 1/dt of the current operator (0 = steady)
     double tol;
@@ -60,6 +68,13 @@ struct State {
     double chk_rr, chk_bb;         // true-residual check
     double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
     int maxit, iters, done, converged, zero_x, flexible;
+    // Transient stepping on the device (transient3d.h:57-84): the time, the
+    // step size of the current operator and of the next step, the stepping
+    // rule, the heat content after the last step and that step's record.
+    double t, dt, dt_next, growth, dt_max, t_end, heat_prev;
+    int refactor;  // 1: the level factors must be rebuilt for inv_dt
+    int steps;
+    StepRec last;
 };
 
 // Conductance arrays of one level (split layout).

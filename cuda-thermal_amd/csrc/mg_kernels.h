@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
     const int q = blockIdx.x * kColsX + threadIdx.x;
     const int j = blockIdx.y * kColsY + threadIdx.y;
     int i;
-    if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
+    if (!st->refactor || j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
     const Col c = make_col(g, i, j);
     const T inv_dt = (T)st->inv_dt;
     T cp_prev = 0;

@@ -279,8 +279,59 @@ __global__ void k_reopen(State* st) {
     st->done = 0;
     st->converged = 0;
 }
-__global__ void k_set_inv_dt(State* st, double inv_dt) { st->inv_dt = inv_dt; }
+__global__ void k_set_inv_dt(State* st, double inv_dt) {
+    st->inv_dt = inv_dt;
+    st->refactor = 1;
+}
 __global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
+
+// Transient stepping on the device (transient3d.h:57-84), so that a step is
+// a replayable graph whose size of step is not a launch argument.
+// k_step_init: t = 0, first dt, stepping rule, heat content of x = 0.
+__global__ void k_step_init(State* st, double dt0, double growth, double dt_max, double t_end) {
+    st->t = 0.0;
+    st->dt = -1.0;
+    st->dt_next = dt0;
+    st->growth = growth;
+    st->dt_max = dt_max;
+    st->t_end = t_end;
+    st->heat_prev = 0.0;
+    st->steps = 0;
+}
+// dt = min(dt, t_end - t) (transient3d.h:60); a changed dt sets the new
+// operator's 1/dt and asks for new factors (set_dt only if dt changed, :61).
+// Then tolerance and flags of the step's solve, as k_begin.
+__global__ void k_step_begin(State* st, double tol, int maxit) {
+    const double dt = fmin(st->dt_next, st->t_end - st->t);
+    st->refactor = dt != st->dt;
+    if (st->refactor) {
+        st->dt = dt;
+        st->inv_dt = 1.0 / dt;
+    }
+    st->tol = tol;
+    st->maxit = maxit;
+    st->iters = 0;
+    st->done = 0;
+    st->converged = 0;
+    st->zero_x = 0;
+    st->relres = 0.0;
+}
+// After the step's energy terms (k_finalize FIN_ENERGY): t += dt, the step's
+// record with q_stored = (sum C T_new - sum C T_old) / dt, and the next dt
+// = min(dt * growth, dt_max) (transient3d.h:73-84).
+__global__ void k_step_end(State* st) {
+    const double dt = st->dt;
+    st->t += dt;
+    st->last.t = st->t;
+    st->last.dt = dt;
+    st->last.iters = st->iters;
+    st->last.T_max = st->e_tmax;
+    st->last.q_out = st->e_out;
+    st->last.q_stored = (st->e_heat - st->heat_prev) / dt;
+    st->heat_prev = st->e_heat;
+    st->dt_next = fmin(dt * st->growth, st->dt_max);
+    st->steps += 1;
+}
 
 // x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).
 __global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
@@ -297,6 +348,7 @@ __global__ __launch_bounds__(256) void k_invdiag(Geo g, Ops<T> o, double* __rest
     int i;
     if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
     const Col c = make_col(g, i, j);
+    if (!st->refactor) return;
     const double inv_dt = st->inv_dt;
     for (int l = 0; l < g.nz; ++l) {
         const long base = (long)l * g.plane, id = base + c.c0;

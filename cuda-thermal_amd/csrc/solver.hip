@@ -86,6 +86,8 @@ struct SolverBase {
     virtual int precondition(const double* r, double* z) = 0;
     virtual int apply_operator(const double* x, double* y) = 0;
     virtual int time_kernel(int which, int reps, double* avg_ms, double* bytes) = 0;
+    virtual int run_transient(const th_transient_opts& o, th_step_record* recs, int rec_cap, int* nsteps,
+                              int* all_converged, double* ms) = 0;
 };
 
 template <class T>
@@ -240,9 +242,13 @@ struct Solver final : SolverBase {
     }
 
     int capture(hipGraphExec_t* exec, bool start) {
+        return capture_fn(exec, [&]() { return start ? enqueue_start() : enqueue_iter(); });
+    }
+    template <class F>
+    int capture_fn(hipGraphExec_t* exec, F&& enqueue) {
         hipGraph_t gr = nullptr;
         HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = start ? enqueue_start() : enqueue_iter();
+        int rc = enqueue();
         hipError_t e = hipStreamEndCapture(s, &gr);
         if (rc) {
             if (gr) (void)hipGraphDestroy(gr);
@@ -262,6 +268,11 @@ struct Solver final : SolverBase {
         inv_dt = dt > 0 ? 1.0 / dt : 0.0;
         k_set_inv_dt<<<1, 1, 0, s>>>(st, inv_dt);
         HIPTRY(hipGetLastError());
+        return enqueue_factors();
+    }
+    // Level factors (MG) or 1/diag (Jacobi) for the state's 1/dt; the kernels
+    // return at once unless the state asks for new factors.
+    int enqueue_factors() {
         if (mg()) {
             for (DevLevel<T>& d : L) {
                 k_factor<T><<<col_grid(2 * d.g.hx, d.g.ny), col_block(), 0, s>>>(d.g, d.ops(), d.finv, st);
@@ -519,12 +530,13 @@ struct Solver final : SolverBase {
     // PCG iterations until the device reports done. Iterations are queued in
     // batches sized from the observed convergence rate, so the host waits
     // once per batch rather than once per iteration; iterations queued past
-    // convergence return at once.
-    int iterate(int maxit) {
+    // convergence return at once. `first`: size of the first batch.
+    int iterate(int maxit, int first = 1) {
         int rc;
         double rate = 0.0;  // relres reduction per iteration, once known
         while (!h_st->done && h_st->iters < maxit) {
-            int k = 1;
+            int k = first;
+            first = 1;
             if (rate > 0.0 && rate < 1.0 && h_st->relres > 0.0) {
                 const double need = std::log(h_st->tol / h_st->relres) / std::log(rate);
                 k = (int)std::max(1.0, std::min(16.0, std::ceil(need) - 1.0));
@@ -547,18 +559,7 @@ struct Solver final : SolverBase {
         k_begin<<<1, 1, 0, s>>>(st, tol, maxit);
         HIPTRY(hipGetLastError());
         if ((rc = run_start()) || (rc = fetch_state())) return rc;
-        if (maxit > 0 && (rc = iterate(maxit))) return rc;
-        // Optional true-residual check: the recurrence ||r|| drifts from
-        // ||b - A x|| by rounding; restart from x with the true residual
-        // until that one is below tol too.
-        for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
-            k_reopen<<<1, 1, 0, s>>>(st);
-            HIPTRY(hipGetLastError());
-            if ((rc = run_start()) || (rc = fetch_state())) return rc;
-            if (h_st->done) break;  // true residual below tol
-            if (h_st->iters >= maxit) break;
-            if ((rc = iterate(maxit))) return rc;
-        }
+        if ((rc = converge(maxit, 1))) return rc;
         k_finish<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, st);
         HIPTRY(hipGetLastError());
         if (out) {
@@ -567,6 +568,112 @@ struct Solver final : SolverBase {
             out->converged = h_st->converged;
         }
         return TH_OK;
+    }
+
+    // Iterations after the start of a solve (h_st fresh), then the optional
+    // true-residual check: the recurrence ||r|| drifts from ||b - A x|| by
+    // rounding; restart from x with the true residual until that one is
+    // below tol too.
+    int converge(int maxit, int first) {
+        int rc;
+        if (maxit > 0 && (rc = iterate(maxit, first))) return rc;
+        for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
+            k_reopen<<<1, 1, 0, s>>>(st);
+            HIPTRY(hipGetLastError());
+            if ((rc = run_start()) || (rc = fetch_state())) return rc;
+            if (h_st->done) break;  // true residual below tol
+            if (h_st->iters >= maxit) break;
+            if ((rc = iterate(maxit))) return rc;
+        }
+        return TH_OK;
+    }
+
+    // ---- transient (transient3d.h:48-90) -----------------------------------
+    // A step is three launch sequences, each replayed as one graph:
+    //   head: advance dt on the device, new factors if dt changed, b = P +
+    //         C/dt x, start of the solve warm-started from x;
+    //   iterations, queued ahead as in a steady solve;
+    //   tail: energy terms of the new x and the step's record.
+    // The host keeps its own copy of t and dt (the same double arithmetic
+    // as the device) to know when to stop, and waits once per step.
+    int enqueue_step_head(double tol, int maxit) {
+        k_step_begin<<<1, 1, 0, s>>>(st, tol, maxit);
+        HIPTRY(hipGetLastError());
+        int rc;
+        if ((rc = enqueue_factors())) return rc;
+        if ((rc = set_rhs_transient())) return rc;
+        return enqueue_start();
+    }
+    int enqueue_step_tail() {
+        const unsigned vb = vec_blocks(n0);
+        k_finish<<<vb, kVecThreads, 0, s>>>(n0, x, st);
+        HIPTRY(hipGetLastError());
+        k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
+        HIPTRY(hipGetLastError());
+        int rc;
+        if ((rc = finalize<3, FIN_ENERGY, 4u>(vb))) return rc;
+        k_step_end<<<1, 1, 0, s>>>(st);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int run_transient(const th_transient_opts& o, th_step_record* recs, int rec_cap, int* nsteps, int* all_converged,
+                      double* ms) override {
+        static_assert(sizeof(StepRec) == sizeof(th_step_record), "step record layout");
+        using clk = std::chrono::high_resolution_clock;
+        int rc;
+        hipGraphExec_t gh = nullptr, gt = nullptr;
+        auto cleanup = [&]() {
+            if (gh) (void)hipGraphExecDestroy(gh);
+            if (gt) (void)hipGraphExecDestroy(gt);
+        };
+        if (opt.use_graphs && mg()) {
+            if ((rc = capture_fn(&gh, [&]() { return enqueue_step_head(o.tol, o.maxit); })) ||
+                (rc = capture_fn(&gt, [&]() { return enqueue_step_tail(); }))) {
+                cleanup();
+                return rc;
+            }
+        }
+        auto body = [&]() -> int {
+            if ((rc = set_x_zero())) return rc;
+            k_step_init<<<1, 1, 0, s>>>(st, o.dt0, o.growth, o.dt_max, o.t_end);
+            HIPTRY(hipGetLastError());
+            HIPTRY(hipStreamSynchronize(s));
+            const auto t0 = clk::now();
+            double t = 0.0, dt = o.dt0;
+            int steps = 0, conv = 1, prev_iters = 2;
+            auto take_record = [&](int k) -> int {  // h_st->last is the record of step k
+                if (!recs || k < 0) return TH_OK;
+                if (k >= rec_cap) return fail(TH_ERR_ARG, "record buffer too small");
+                std::memcpy(&recs[k], &h_st->last, sizeof(StepRec));
+                return TH_OK;
+            };
+            while (t < o.t_end * (1 - 1e-12) && (o.max_steps <= 0 || steps < o.max_steps)) {
+                dt = std::min(dt, o.t_end - t);
+                if (gh) HIPTRY(hipGraphLaunch(gh, s));
+                else if ((rc = enqueue_step_head(o.tol, o.maxit))) return rc;
+                for (int q = 0; q < std::min(prev_iters, o.maxit); ++q)
+                    if ((rc = run_iter())) return rc;
+                if ((rc = fetch_state())) return rc;
+                if ((rc = take_record(steps - 1))) return rc;
+                if ((rc = converge(o.maxit, 1))) return rc;
+                if (!h_st->converged) conv = 0;
+                prev_iters = std::max(1, h_st->iters);
+                if (gt) HIPTRY(hipGraphLaunch(gt, s));
+                else if ((rc = enqueue_step_tail())) return rc;
+                t += dt;
+                dt = std::min(dt * o.growth, o.dt_max);
+                ++steps;
+            }
+            if ((rc = fetch_state())) return rc;
+            if ((rc = take_record(steps - 1))) return rc;
+            if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+            *nsteps = steps;
+            if (all_converged) *all_converged = conv;
+            return TH_OK;
+        };
+        rc = body();
+        cleanup();
+        return rc;
     }
 
     int true_relres(double* out) override {
@@ -862,55 +969,16 @@ void th_transient_default_opts(th_transient_opts* o) {
     o->max_steps = 0;
 }
 
-// transient3d.h:48-90, with the energy terms on the device: heat content
-// sum C T before and after the step gives the stored heat, heat out through
-// the boundaries, and T_max.
+// transient3d.h:48-90, stepped on the device (Solver::run_transient).
 int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* recs, int rec_cap, int* nsteps,
                      double* T_final, int* all_converged, double* solve_ms) {
     TH_S(s);
     if (!o || !nsteps) return fail(TH_ERR_ARG, "null argument");
-    using clk = std::chrono::high_resolution_clock;
-    SolverBase* S = s->impl;
-    int rc;
-    if ((rc = S->set_x_zero())) return rc;
-    double heat_old = 0, q, tmax, heat_new;
-    if ((rc = S->energy(&heat_old, &q, &tmax))) return rc;
-    double t = 0, dt = o->dt0, last_dt = -1, ms = 0;
-    int steps = 0;
-    int conv_all = 1;
-    while (t < o->t_end * (1 - 1e-12) && (o->max_steps <= 0 || steps < o->max_steps)) {
-        dt = std::min(dt, o->t_end - t);
-        const auto ts = clk::now();
-        if (dt != last_dt) {
-            if ((rc = S->set_dt(dt))) return rc;
-            last_dt = dt;
-        }
-        if ((rc = S->set_rhs_transient())) return rc;
-        th_stats st{};
-        if ((rc = S->solve(o->tol, o->maxit, &st))) return rc;
-        (void)hipStreamSynchronize(S->stream());
-        ms += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
-        if ((rc = S->energy(&heat_new, &q, &tmax))) return rc;
-        t += dt;
-        if (recs) {
-            if (steps >= rec_cap) return fail(TH_ERR_ARG, "record buffer too small");
-            th_step_record& R = recs[steps];
-            R.t = t;
-            R.dt = dt;
-            R.iters = st.iters;
-            R.T_max = tmax;
-            R.q_out = q;
-            R.q_stored = (heat_new - heat_old) / dt;
-        }
-        heat_old = heat_new;
-        if (!st.converged) conv_all = 0;
-        dt = std::min(dt * o->growth, o->dt_max);
-        ++steps;
-    }
-    *nsteps = steps;
-    if (all_converged) *all_converged = conv_all;
-    if (solve_ms) *solve_ms = ms;
-    if (T_final) return S->get_x(T_final);
+    if (!(o->dt0 > 0) || !(o->growth > 0) || !(o->dt_max > 0) || o->maxit < 0)
+        return fail(TH_ERR_ARG, "transient options out of range");
+    int rc = s->impl->run_transient(*o, recs, rec_cap, nsteps, all_converged, solve_ms);
+    if (rc) return rc;
+    if (T_final) return s->impl->get_x(T_final);
     return TH_OK;
 }
 

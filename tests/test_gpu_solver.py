@@ -167,12 +167,54 @@ def test_transient_vs_oracle(graphs):
     s = T.Solver(m, graphs=graphs)
     g = T.run_transient(s, t_end=0.02, growth=1.2, tol=1e-9)
     assert g["converged"] and len(g["t"]) == len(ref["t"])
+    assert np.array_equal(g["t"], ref["t"]) and np.array_equal(g["dt"], ref["dt"])
     assert np.abs(g["iters"] - ref["iters"]).max() <= 1
     assert np.abs(g["T"] - ref["T"]).max() < 1e-7 * ref["T"].max()
     assert np.allclose(g["T_max"], ref["T_max"], rtol=1e-7)
     P = m.total_power()
     eb = np.abs(g["q_out"] + g["q_stored"] - P) / P
     assert eb.max() < 1e-6
+
+
+def test_transient_step_rule_vs_oracle():
+    """The 100-step rule of the 800x800x10 transient (dt0 = 10 us growing 10%
+    per step, no cap) on a 48x48 grid against the oracle, step by step."""
+    m = hotspot(48, at=(5e-3, 5e-3))
+    a = m.arrays()
+    kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    ref = O.run_transient(a, flexible=True, **kw)
+    g = T.run_transient(T.Solver(m), **kw)
+    assert g["converged"] and len(g["t"]) == 100
+    assert np.array_equal(g["t"], ref["t"]) and np.array_equal(g["dt"], ref["dt"])
+    assert np.isclose(g["t"][-1], 1e-5 * (1.1 ** 100 - 1) / 0.1, rtol=1e-12)
+    assert np.abs(g["iters"] - ref["iters"]).max() <= 1
+    assert np.abs(g["T"] - ref["T"]).max() < 1e-7 * ref["T"].max()
+    assert np.allclose(g["T_max"], ref["T_max"], rtol=1e-7)
+    assert np.allclose(g["q_out"], ref["q_out"], rtol=1e-7, atol=1e-9 * m.total_power())
+
+
+def test_transient_800x800x10_100_steps():
+    """The 800x800x10 transient from ambient, 100 backward-Euler steps:
+    per-step energy balance, mixed vs FP64 steps, and the final field
+    against the steady state (the heat-sink mode, RC ~ 0.18 s, leaves a few
+    1e-4 of the rise after 1.38 s)."""
+    m = hotspot(800, at=(5e-3, 5e-3))
+    P = m.total_power()
+    kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        g = T.run_transient(T.Solver(m, precision=prec), **kw)
+        assert g["converged"] and len(g["t"]) == 100
+        eb = np.abs(g["q_out"] + g["q_stored"] - P) / P
+        assert eb.max() < 1e-6, (prec, eb.max())
+        res[prec] = g
+    assert np.abs(res["mixed"]["T"] - res["fp64"]["T"]).max() < 5e-4
+    steady = T.steady_solve(m, tol=1e-10, precision="fp64")["T"]
+    dev = np.abs(res["fp64"]["T"] - steady).max() / steady.max()
+    assert 1e-5 < dev < 2e-3, dev
+    # the deviation decays: the last steps approach the steady state
+    assert np.all(np.diff(res["fp64"]["T_max"][-10:]) > 0)
+    assert res["fp64"]["T_max"][-1] < steady.max()
 
 
 def test_graphs_match_direct_launches():
