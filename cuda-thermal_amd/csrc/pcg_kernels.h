@@ -396,6 +396,87 @@ __global__ __launch_bounds__(kVecThreads) void k_energy(Geo g, Ops<T> o, const d
     block_partials<3, 4u>(v, part);
 }
 
+// ---- initial guess from a coarse field ----------------------------------------
+
+// Plane-to-channel map of the lift: plane l of the solver's grid takes
+// (1 - w[l]) f[lo[l]] + w[l] f[lo[l] + 1] of the coarse field.
+struct LiftMap {
+    int lo[64];
+    float w[64];
+};
+
+// x(i, j, l) = scale * the coarse field at the centre of cell (i, j),
+// bilinear over the Hc x Wc coarse cells (clamped at the edges), then
+// linear in z through the map.
+__global__ void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C, LiftMap zm, double scale,
+                       double* __restrict__ x) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)g.nx * g.ny * g.nz) return;
+    const int i = (int)(k % g.nx);
+    const long rest = k / g.nx;
+    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
+    const double xc = fmin(fmax((i + 0.5) * Wc / g.nx - 0.5, 0.0), (double)(Wc - 1));
+    const double yc = fmin(fmax((j + 0.5) * Hc / g.ny - 0.5, 0.0), (double)(Hc - 1));
+    const int x0 = (int)xc, y0 = (int)yc;
+    const int x1 = x0 + 1 < Wc ? x0 + 1 : x0, y1 = y0 + 1 < Hc ? y0 + 1 : y0;
+    const double fx = xc - x0, fy = yc - y0;
+    const int lo = zm.lo[l];
+    const double w = zm.w[l];
+    auto at = [&](int yy, int xx) {
+        const float* p = f + ((long)yy * Wc + xx) * C + lo;
+        return w > 0.0 ? (1.0 - w) * p[0] + w * p[1] : (double)p[0];
+    };
+    const double v = (1.0 - fy) * ((1.0 - fx) * at(y0, x0) + fx * at(y0, x1)) +
+                     fy * ((1.0 - fx) * at(y1, x0) + fx * at(y1, x1));
+    x[(long)l * g.plane + g.pos(i, j)] = scale * v;
+}
+
+// One colour of a column line relaxation of the steady operator in FP64:
+// each column of `colour` is solved exactly (Thomas) with its lateral
+// neighbours fixed at their current values and right-hand side b.
+template <class T, int NZ>
+__global__ __launch_bounds__(256) void k_line_relax(Geo g, Ops<T> o, const double* __restrict__ b,
+                                                    double* __restrict__ x, int colour) {
+    const int t = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    if (j >= g.ny || t >= g.hx) return;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= g.nx) return;
+    const Col c = make_col(g, i, j);
+    double cp[NZ], dp[NZ];
+    double pcp = 0.0, pdp = 0.0;
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < g.nz) {
+            const long base = (long)l * g.plane, id = base + c.c0;
+            const double Gw = (double)o.gE[base + c.pw], Ge = (double)o.gE[id];
+            const double Gs = (double)o.gN[base + c.ps], Gn = (double)o.gN[id];
+            const double Gu = (double)o.gU[id], Gd = (l > 0) ? (double)o.gU[id - g.plane] : (double)o.gB[c.c0];
+            double d = 0.0, rhs = b[id];
+            if (c.hw) d += Gw, rhs += Gw * x[base + c.pw];
+            if (c.he) d += Ge, rhs += Ge * x[base + c.pe];
+            if (c.hs) d += Gs, rhs += Gs * x[base + c.ps];
+            if (c.hn) d += Gn, rhs += Gn * x[base + c.pn];
+            d += Gu + Gd;
+            const double lower = (l > 0) ? -Gd : 0.0, upper = (l < g.nz - 1) ? -Gu : 0.0;
+            const double den = d - lower * pcp;
+            cp[l] = upper / den;
+            dp[l] = (rhs - lower * pdp) / den;
+            pcp = cp[l];
+            pdp = dp[l];
+        }
+    }
+    double xu = 0.0;
+#pragma unroll
+    for (int l = NZ - 1; l >= 0; --l) {
+        if (l < g.nz) {
+            const double v = (l == g.nz - 1) ? dp[l] : dp[l] - cp[l] * xu;
+            x[(long)l * g.plane + c.c0] = v;
+            xu = v;
+        }
+    }
+}
+
 // Natural <-> split layout. Padding slots of `out` are left untouched
 // (they were cleared at allocation).
 template <class S, class D>

@@ -88,6 +88,9 @@ struct SolverBase {
     virtual int time_kernel(int which, int reps, double* avg_ms, double* bytes) = 0;
     virtual int run_transient(const th_transient_opts& o, th_step_record* recs, int rec_cap, int* nsteps,
                               int* all_converged, double* ms) = 0;
+    virtual int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
+                             int* used, double* rp, double* rl) = 0;
+    virtual int nz() const = 0;
 };
 
 template <class T>
@@ -676,6 +679,43 @@ struct Solver final : SolverBase {
         return rc;
     }
 
+    // Initial guess from a coarse field with the plain warm start as the
+    // fallback (th_solver_set_x_lifted). Ap holds the plain x meanwhile.
+    int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
+                     int* used, double* rp, double* rl) override {
+        int rc;
+        double r_plain = 0.0, r_lift = 0.0;
+        if ((rc = true_relres(&r_plain))) return rc;
+        HIPTRY(hipMemcpyAsync(Ap, x, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, x);
+        HIPTRY(hipGetLastError());
+        const Ops<T> o = L[0].ops();
+        for (int k = 0; k < passes; ++k)
+            for (int colour = 0; colour < 2; ++colour) {
+                const dim3 grid = col_grid(g0.hx, g0.ny);
+#define TH_RELAX(NZ) k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, x, colour)
+                if (g0.nz <= 4) TH_RELAX(4);
+                else if (g0.nz <= 8) TH_RELAX(8);
+                else if (g0.nz <= 12) TH_RELAX(12);
+                else if (g0.nz <= 16) TH_RELAX(16);
+                else if (g0.nz <= 24) TH_RELAX(24);
+                else if (g0.nz <= 32) TH_RELAX(32);
+                else if (g0.nz <= 48) TH_RELAX(48);
+                else TH_RELAX(64);
+#undef TH_RELAX
+                HIPTRY(hipGetLastError());
+            }
+        if ((rc = true_relres(&r_lift))) return rc;
+        const bool keep = r_lift < r_plain;
+        if (!keep) HIPTRY(hipMemcpyAsync(x, Ap, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (used) *used = keep;
+        if (rp) *rp = r_plain;
+        if (rl) *rl = r_lift;
+        return TH_OK;
+    }
+    int nz() const override { return g0.nz; }
+
     int true_relres(double* out) override {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, x, nullptr, nullptr, nullptr,
@@ -967,6 +1007,29 @@ void th_transient_default_opts(th_transient_opts* o) {
     o->tol = 1e-9;
     o->maxit = 50000;
     o->max_steps = 0;
+}
+
+int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc, const double* zc_coarse,
+                           const double* zc_fine, double scale, int passes, int* used, double* relres_plain,
+                           double* relres_lift) {
+    TH_S(s);
+    if (!field || !zc_coarse || !zc_fine) return fail(TH_ERR_ARG, "null argument");
+    if (Hc < 1 || Wc < 1 || nzc < 1 || nzc > C || passes < 0) return fail(TH_ERR_ARG, "bad coarse field shape");
+    const int nz = s->impl->nz();
+    LiftMap zm{};
+    for (int l = 0; l < nz; ++l) {
+        const double z = zc_fine[l];
+        int k = 0;
+        while (k + 1 < nzc && zc_coarse[k + 1] <= z) ++k;
+        if (k + 1 >= nzc || z <= zc_coarse[0]) {  // outside the coarse centres: nearest plane
+            zm.lo[l] = z <= zc_coarse[0] ? 0 : nzc - 1;
+            zm.w[l] = 0.0f;
+        } else {
+            zm.lo[l] = k;
+            zm.w[l] = (float)((z - zc_coarse[k]) / (zc_coarse[k + 1] - zc_coarse[k]));
+        }
+    }
+    return s->impl->set_x_lifted(field, Hc, Wc, C, zm, scale, passes, used, relres_plain, relres_lift);
 }
 
 // transient3d.h:48-90, stepped on the device (Solver::run_transient).

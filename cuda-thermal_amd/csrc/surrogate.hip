@@ -1,0 +1,984 @@
+// surrogate.hip - the learned warm start of the solver: a small U-Net that
+// maps the power density of the two dies and the TSV copper fraction on a
+// 256 x 256 grid to the temperature rise of the 10 planes of the two-tier
+// stack. Forward, backward and Adam kernels for gfx950 (BF16 MFMA with FP32
+// accumulation), the training loop in C++, and the C ABI
+// (include/thermal_hip.h). The reference has no learned part; the network
+// is the one the project's benchmark description specifies (4 down/up
+// levels, 32-64 channels, ~230k parameters).
+//
+// Tensors are NHWC with 16, 32 or 64 channels, pixels flattened over
+// (image, row, column). A 3x3 convolution is an implicit GEMM over
+// K = 9 taps x Cin: each wavefront owns 16 consecutive pixels and all output
+// channels, loads the 16 x K input slice of one tap straight from global
+// memory (neighbouring taps and waves hit the same cache lines) and issues
+// one MFMA per 16 output channels: v_mfma_f32_16x16x32_bf16 when Cin is a
+// multiple of 32, v_mfma_f32_16x16x16_bf16 for the 16-channel input layer.
+// The data gradient is the same kernel on flipped, transposed weights; the
+// weight gradient is an MFMA GEMM over pixels, one wave per (pixel chunk,
+// tap), with the bias gradient as an extra tap whose input is 1.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "model.h"
+
+
+namespace th {
+namespace sur {
+
+using bf = unsigned short;  // bfloat16 storage
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float bf2f(bf h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ bf f2bf(float f) {  // nearest even
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf)(u >> 16);
+}
+static bf host_f2bf(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf)(u >> 16);
+}
+
+// (level, cin, cout, kernel size, relu); the input layer's 16 channels carry
+// 3 features, the head's 16 outputs carry 10 planes.
+struct LayerSpec {
+    int level, cin, cout, ks, relu;
+};
+constexpr int kLayers = 12;
+static const LayerSpec kSpec[kLayers] = {
+    {0, 16, 32, 3, 1}, {0, 32, 32, 3, 1}, {1, 32, 32, 3, 1}, {2, 32, 64, 3, 1},
+    {2, 64, 64, 3, 1}, {3, 64, 64, 3, 1}, {4, 64, 64, 3, 1}, {3, 64, 64, 3, 1},
+    {2, 64, 32, 3, 1}, {1, 32, 32, 3, 1}, {0, 32, 32, 3, 1}, {0, 32, 16, 1, 0}};
+enum { E0A, E0B, E1A, E2A, E2B, E3A, B0, D3, D2, D1, D0, HEAD };
+constexpr int kIn = 3, kOut = 10, kInStore = 4;
+
+static long layer_params(const LayerSpec& l) { return (long)l.cout * l.ks * l.ks * l.cin + l.cout; }
+static long total_params() {
+    long n = 0;
+    for (const LayerSpec& l : kSpec) n += layer_params(l);
+    return n;
+}
+
+// ---- kernels -----------------------------------------------------------------
+
+enum { EPI_RELU = 0, EPI_LINEAR = 1, EPI_F32 = 2 };
+
+// Y = epi(conv(X, W) + b); Wf[tap][cout][cin].
+template <int KS, int CIN, int COUT, int EPI>
+__global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf* __restrict__ Wf,
+                                              const float* __restrict__ bias, void* __restrict__ Yv, long npix,
+                                              int H, int W) {
+    constexpr int KC = (CIN % 32 == 0) ? 32 : 16;  // K of one MFMA
+    constexpr int NT = COUT / 16;
+    const int lane = threadIdx.x & 63;
+    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (p0 >= npix) return;
+    const int m = lane & 15, q = lane >> 4;
+    const long pm = p0 + m;
+    const bool valid = pm < npix;
+    const long hw = (long)H * W;
+    const long img = valid ? pm / hw : 0;
+    const int rem = valid ? (int)(pm - img * hw) : 0;
+    const int y = rem / W, x = rem - (rem / W) * W;
+    f4v acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < KS * KS; ++tap) {
+        const int yy = y + tap / KS - KS / 2, xx = x + tap % KS - KS / 2;
+        const bool inb = valid && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const bf* xp = X + (img * hw + (long)yy * W + xx) * CIN;
+        const bf* wp = Wf + ((long)tap * COUT + m) * CIN;
+#pragma unroll
+        for (int c0 = 0; c0 < CIN; c0 += KC) {
+            if constexpr (KC == 32) {
+                const u4v av = inb ? *(const u4v*)(xp + c0 + 8 * q) : u4v{0u, 0u, 0u, 0u};
+                const b8v a = __builtin_bit_cast(b8v, av);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const u4v bv = *(const u4v*)(wp + (long)t * 16 * CIN + c0 + 8 * q);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(b8v, bv), acc[t], 0, 0, 0);
+                }
+            } else {
+                const u2v av = inb ? *(const u2v*)(xp + c0 + 4 * q) : u2v{0u, 0u};
+                const s4v a = __builtin_bit_cast(s4v, av);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const u2v bv = *(const u2v*)(wp + (long)t * 16 * CIN + c0 + 4 * q);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, __builtin_bit_cast(s4v, bv), acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane (m, q) holds rows (pixels) 4q + i, column (channel) m of each tile
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + m;
+        const float bb = (EPI == EPI_LINEAR) ? 0.f : bias[co];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long pr = p0 + 4 * q + i;
+            if (pr >= npix) continue;
+            float v = acc[t][i] + bb;
+            if (EPI == EPI_RELU) v = fmaxf(v, 0.f);
+            if (EPI == EPI_F32) ((float*)Yv)[pr * COUT + co] = v;
+            else ((bf*)Yv)[pr * COUT + co] = f2bf(v);
+        }
+    }
+}
+
+// Partial weight gradients of one pixel chunk for one tap:
+// part[chunk][tap][cout][cin] = sum_p GZ[p][co] X[p + off(tap)][ci]; tap ==
+// KS*KS is the bias (input 1 in channel 0).
+template <int KS, int CIN, int COUT>
+__global__ __launch_bounds__(64) void k_wgrad(const bf* __restrict__ X, const bf* __restrict__ GZ,
+                                              float* __restrict__ part, long npix, int H, int W, long chunk) {
+    constexpr int KK = KS * KS, NT = COUT / 16, NU = CIN / 16;
+    const int lane = threadIdx.x, m = lane & 15, q = lane >> 4;
+    const int tap = blockIdx.y;
+    const int dy = tap / KS - KS / 2, dx = tap % KS - KS / 2;
+    const long pb = (long)blockIdx.x * chunk, pe = (pb + chunk < npix) ? pb + chunk : npix;
+    const long hw = (long)H * W;
+    f4v acc[NT][NU];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int u = 0; u < NU; ++u) acc[t][u] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (long p0 = pb; p0 < pe; p0 += 16) {
+        s4v a[NT], b[NU];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long p = p0 + 4 * q + j;
+            const bool ok = p < pe;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) a[t][j] = ok ? (short)GZ[p * COUT + t * 16 + m] : (short)0;
+            if (tap == KK) {
+#pragma unroll
+                for (int u = 0; u < NU; ++u) b[u][j] = (ok && u == 0 && m == 0) ? (short)0x3F80 : (short)0;
+            } else {
+                const long img = ok ? p / hw : 0;
+                const int rem = ok ? (int)(p - img * hw) : 0;
+                const int yy = rem / W + dy, xx = rem % W + dx;
+                const bool inb = ok && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                const bf* xp = X + (img * hw + (long)yy * W + xx) * CIN + m;
+#pragma unroll
+                for (int u = 0; u < NU; ++u) b[u][j] = inb ? (short)xp[u * 16] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int u = 0; u < NU; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+    float* out = part + ((long)blockIdx.x * (KK + 1) + tap) * COUT * CIN;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[(t * 16 + 4 * q + i) * CIN + u * 16 + m] = acc[t][u][i];
+}
+
+// Sums the chunks' partials into the canonical gradient layout
+// gW[cout][tap][cin], gb[cout].
+__global__ void k_reduce_grad(const float* __restrict__ part, int nchunks, int KK, int COUT, int CIN,
+                              float* __restrict__ gW, float* __restrict__ gb) {
+    const int n = COUT * KK * CIN + COUT;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    long src;
+    if (k < COUT * KK * CIN) {
+        const int co = k / (KK * CIN), tap = (k / CIN) % KK, ci = k % CIN;
+        src = ((long)tap * COUT + co) * CIN + ci;
+    } else {
+        src = ((long)KK * COUT + (k - COUT * KK * CIN)) * CIN;
+    }
+    const long stride = (long)(KK + 1) * COUT * CIN;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += part[c * stride + src];
+    if (k < COUT * KK * CIN) gW[k] = s;
+    else gb[k - COUT * KK * CIN] = s;
+}
+
+__global__ void k_relu_bwd(bf* __restrict__ g, const bf* __restrict__ y, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && !(bf2f(y[k]) > 0.f)) g[k] = 0;
+}
+
+__global__ void k_pool(const bf* __restrict__ X, bf* __restrict__ Y, long nout, int Ho, int Wo, int C) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nout) return;
+    const int c = (int)(k % C);
+    const long pix = k / C;
+    const int xo = (int)(pix % Wo);
+    const long r = pix / Wo;
+    const int yo = (int)(r % Ho);
+    const long img = r / Ho;
+    const int W = 2 * Wo;
+    const long base = ((img * 2 * Ho + 2 * yo) * (long)W + 2 * xo) * C + c;
+    const float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (long)W * C]) + bf2f(X[base + (long)W * C + C]);
+    Y[k] = f2bf(0.25f * s);
+}
+
+// Y = up(Xc) + S (nearest, x2).
+__global__ void k_upadd(const bf* __restrict__ Xc, const bf* __restrict__ S, bf* __restrict__ Y, long n, int H, int W,
+                        int C) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = (int)(k % C);
+    const long pix = k / C;
+    const int x = (int)(pix % W);
+    const long r = pix / W;
+    const int y = (int)(r % H);
+    const long img = r / H;
+    const long ck = ((img * (H / 2) + y / 2) * (long)(W / 2) + x / 2) * C + c;
+    Y[k] = f2bf(bf2f(Xc[ck]) + bf2f(S[k]));
+}
+
+// Gc = sum over the 2x2 children of G.
+__global__ void k_upbwd(const bf* __restrict__ G, bf* __restrict__ Gc, long nc, int Hc, int Wc, int C) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const int c = (int)(k % C);
+    const long pix = k / C;
+    const int xo = (int)(pix % Wc);
+    const long r = pix / Wc;
+    const int yo = (int)(r % Hc);
+    const long img = r / Hc;
+    const int W = 2 * Wc;
+    const long base = ((img * 2 * Hc + 2 * yo) * (long)W + 2 * xo) * C + c;
+    Gc[k] = f2bf(bf2f(G[base]) + bf2f(G[base + C]) + bf2f(G[base + (long)W * C]) + bf2f(G[base + (long)W * C + C]));
+}
+
+// G = 0.25 up(Gp) + Gs: gradient of a skip tensor that also fed a pool.
+__global__ void k_poolbwd_add(const bf* __restrict__ Gp, const bf* __restrict__ Gs, bf* __restrict__ G, long n, int H,
+                              int W, int C) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = (int)(k % C);
+    const long pix = k / C;
+    const int x = (int)(pix % W);
+    const long r = pix / W;
+    const int y = (int)(r % H);
+    const long img = r / H;
+    const long ck = ((img * (H / 2) + y / 2) * (long)(W / 2) + x / 2) * C + c;
+    G[k] = f2bf(0.25f * bf2f(Gp[ck]) + bf2f(Gs[k]));
+}
+
+// Network input of a batch from the stored features: 16 channels, 3 used.
+__global__ void k_gather(const bf* __restrict__ ds, const int* __restrict__ idx, bf* __restrict__ X0, long n, long hw) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = (int)(k & 15);
+    const long pix = k >> 4;
+    const long img = pix / hw, p = pix - img * hw;
+    X0[k] = c < kIn ? ds[((long)idx[img] * hw + p) * kInStore + c] : (bf)0;
+}
+
+// MSE over the 10 used outputs: gradient 2 (out - tgt) / count in BF16 and
+// per-block partial sums of the squared error. idx == nullptr: targets are
+// batch-contiguous.
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, const float* __restrict__ tgt,
+                                             const int* __restrict__ idx, long hw, long npix, double inv_cnt,
+                                             bf* __restrict__ g, double* __restrict__ part) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (k < npix * 16) {
+        const int c = (int)(k & 15);
+        const long pix = k >> 4;
+        float gv = 0.f;
+        if (c < kOut) {
+            const long img = pix / hw, p = pix - img * hw;
+            const long t = ((idx ? (long)idx[img] : img) * hw + p) * kOut + c;
+            const float d = out[k] - tgt[t];
+            e = (double)d * d;
+            gv = (float)(2.0 * d * inv_cnt);
+        }
+        g[k] = f2bf(gv);
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_to(const double* __restrict__ part, int n, double scale,
+                                                double* __restrict__ dst) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) s += part[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = sh[0] * scale;
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    w[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + eps);
+}
+
+// BF16 copies of one layer's weights: Wf[tap][cout][cin] for the forward
+// pass, Wb[KK-1-tap][cin][cout] (flipped, transposed) for the data gradient.
+__global__ void k_pack(const float* __restrict__ Wc, int KK, int COUT, int CIN, bf* __restrict__ Wf,
+                       bf* __restrict__ Wb) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= COUT * KK * CIN) return;
+    const int co = k / (KK * CIN), tap = (k / CIN) % KK, ci = k % CIN;
+    const bf v = f2bf(Wc[k]);
+    Wf[((long)tap * COUT + co) * CIN + ci] = v;
+    Wb[((long)(KK - 1 - tap) * CIN + ci) * COUT + co] = v;
+}
+
+// ---- host ------------------------------------------------------------------
+
+#define SURTRY(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(TH_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static unsigned grid1(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {  // Box-Muller
+        const double u1 = std::max(uniform(), 1e-300), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+struct Net {
+    int H = 0, W = 0, Bcap = 0;
+    hipStream_t s = nullptr;
+    long nparam = 0;
+    long off[kLayers];
+    float *prm = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    bf *Wf[kLayers] = {}, *Wb[kLayers] = {};
+    long adam_t = 0;
+    // activations and their gradients (sized for Bcap)
+    bf *x0 = nullptr, *a0, *s0, *p0, *s1, *p1, *a2, *s2, *p2, *s3, *p3, *bb, *u3, *c3, *u2, *c2, *u1, *c1, *u0, *c0;
+    float* out = nullptr;
+    bf *g_out, *g_c0, *g_u0, *g_c1, *g_u1, *g_c2, *g_u2, *g_c3, *g_u3, *g_bb, *g_p3, *g_s3, *g_p2, *g_s2, *g_a2,
+        *g_p1, *g_s1, *g_p0, *g_s0, *g_a0;
+    float* wpart = nullptr;
+    long wpart_n = 0;
+    double* lpart = nullptr;
+    double* dloss = nullptr;  // per-step losses (device)
+    int dloss_cap = 0;
+    float* tgt_b = nullptr;  // batch-contiguous targets (gradient checks)
+    int* d_idx = nullptr;
+    // dataset
+    bf* ds_in = nullptr;
+    float* ds_tgt = nullptr;
+    int ds_cap = 0, ds_n = 0;
+    std::vector<void*> act_mem, own;
+
+    long hw() const { return (long)H * W; }
+    long npix(int level, int B) const { return (long)B * (H >> level) * (W >> level); }
+
+    ~Net() {
+        if (s) (void)hipStreamSynchronize(s);
+        free_acts();
+        for (void* p : own) (void)hipFree(p);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void free_acts() {
+        for (void* p : act_mem) (void)hipFree(p);
+        act_mem.clear();
+        Bcap = 0;
+    }
+    template <class U>
+    int dalloc(U** p, size_t n, std::vector<void*>& list) {
+        void* q = nullptr;
+        SURTRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(U)));
+        list.push_back(q);
+        SURTRY(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(U), s));
+        *p = (U*)q;
+        return TH_OK;
+    }
+
+    int init(int h, int w, uint64_t seed) {
+        H = h;
+        W = w;
+        SURTRY(hipStreamCreate(&s));
+        nparam = 0;
+        for (int k = 0; k < kLayers; ++k) {
+            off[k] = nparam;
+            nparam += layer_params(kSpec[k]);
+        }
+        int rc;
+        for (float** p : {&prm, &grad, &m, &v})
+            if ((rc = dalloc(p, nparam, own))) return rc;
+        for (int k = 0; k < kLayers; ++k) {
+            const long nw = (long)kSpec[k].cout * kSpec[k].ks * kSpec[k].ks * kSpec[k].cin;
+            if ((rc = dalloc(&Wf[k], nw, own)) || (rc = dalloc(&Wb[k], nw, own))) return rc;
+        }
+        // wgrad partials: at most kMaxChunks chunks of (taps + 1) x cout x cin
+        wpart_n = 0;
+        for (const LayerSpec& l : kSpec)
+            wpart_n = std::max(wpart_n, (long)kMaxChunks * (l.ks * l.ks + 1) * l.cout * l.cin);
+        if ((rc = dalloc(&wpart, wpart_n, own))) return rc;
+        // He-normal weights (fan-in counts the used input channels), zero bias
+        std::vector<float> h0(nparam, 0.f);
+        SplitMix rng{seed};
+        for (int k = 0; k < kLayers; ++k) {
+            const LayerSpec& l = kSpec[k];
+            const int cin_used = (k == E0A) ? kIn : l.cin;
+            const double sd = (k == HEAD) ? std::sqrt(1.0 / l.cin) : std::sqrt(2.0 / (l.ks * l.ks * cin_used));
+            const int KK = l.ks * l.ks;
+            for (int co = 0; co < l.cout; ++co)
+                for (int t = 0; t < KK; ++t)
+                    for (int ci = 0; ci < l.cin; ++ci)
+                        h0[off[k] + ((long)co * KK + t) * l.cin + ci] =
+                            (ci < cin_used && (k != HEAD || co < kOut)) ? (float)(sd * rng.normal()) : 0.f;
+        }
+        return set_params(h0.data());
+    }
+    static constexpr int kMaxChunks = 512;
+
+    int set_params(const float* host) {
+        SURTRY(hipMemcpyAsync(prm, host, nparam * sizeof(float), hipMemcpyHostToDevice, s));
+        SURTRY(hipMemsetAsync(m, 0, nparam * sizeof(float), s));
+        SURTRY(hipMemsetAsync(v, 0, nparam * sizeof(float), s));
+        adam_t = 0;
+        int rc = pack();
+        if (rc) return rc;
+        SURTRY(hipStreamSynchronize(s));
+        return TH_OK;
+    }
+    int pack() {
+        for (int k = 0; k < kLayers; ++k) {
+            const LayerSpec& l = kSpec[k];
+            const int KK = l.ks * l.ks;
+            k_pack<<<grid1((long)l.cout * KK * l.cin), 256, 0, s>>>(prm + off[k], KK, l.cout, l.cin, Wf[k], Wb[k]);
+            SURTRY(hipGetLastError());
+        }
+        return TH_OK;
+    }
+
+    int ensure_batch(int B) {
+        if (B <= Bcap) return TH_OK;
+        SURTRY(hipStreamSynchronize(s));
+        free_acts();
+        const long n0 = npix(0, B), n1 = npix(1, B), n2 = npix(2, B), n3 = npix(3, B), n4 = npix(4, B);
+        struct A {
+            bf** p;
+            long n;
+        };
+        const A acts[] = {{&x0, n0 * 16}, {&a0, n0 * 32}, {&s0, n0 * 32}, {&p0, n1 * 32}, {&s1, n1 * 32},
+                          {&p1, n2 * 32}, {&a2, n2 * 64}, {&s2, n2 * 64}, {&p2, n3 * 64}, {&s3, n3 * 64},
+                          {&p3, n4 * 64}, {&bb, n4 * 64}, {&u3, n3 * 64}, {&c3, n3 * 64}, {&u2, n2 * 64},
+                          {&c2, n2 * 32}, {&u1, n1 * 32}, {&c1, n1 * 32}, {&u0, n0 * 32}, {&c0, n0 * 32},
+                          {&g_out, n0 * 16}, {&g_c0, n0 * 32}, {&g_u0, n0 * 32}, {&g_c1, n1 * 32}, {&g_u1, n1 * 32},
+                          {&g_c2, n2 * 32}, {&g_u2, n2 * 64}, {&g_c3, n3 * 64}, {&g_u3, n3 * 64}, {&g_bb, n4 * 64},
+                          {&g_p3, n4 * 64}, {&g_s3, n3 * 64}, {&g_p2, n3 * 64}, {&g_s2, n2 * 64}, {&g_a2, n2 * 64},
+                          {&g_p1, n2 * 32}, {&g_s1, n1 * 32}, {&g_p0, n1 * 32}, {&g_s0, n0 * 32}, {&g_a0, n0 * 32}};
+        int rc;
+        for (const A& a : acts)
+            if ((rc = dalloc(a.p, a.n, act_mem))) return rc;
+        if ((rc = dalloc(&out, n0 * 16, act_mem)) || (rc = dalloc(&tgt_b, n0 * kOut, act_mem)) ||
+            (rc = dalloc(&lpart, grid1(n0 * 16), act_mem)) || (rc = dalloc(&d_idx, B, act_mem)))
+            return rc;
+        Bcap = B;
+        return TH_OK;
+    }
+
+    // ---- launches ---------------------------------------------------------
+    int conv(int k, const bf* in, void* outp, int B, int epi) {
+        const LayerSpec& l = kSpec[k];
+        const long n = npix(l.level, B);
+        const int h = H >> l.level, w = W >> l.level;
+        const unsigned g = grid1(n, 64);
+        const float* bias = prm + off[k] + (long)l.cout * l.ks * l.ks * l.cin;
+#define SUR_CONV(KS, CI, CO, EPI) \
+    k_conv<KS, CI, CO, EPI><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w)
+        if (epi == EPI_F32) {
+            if (l.ks == 1 && l.cin == 32 && l.cout == 16) SUR_CONV(1, 32, 16, EPI_F32);
+            else return fail(TH_ERR_ARG, "surrogate: no FP32-output conv for this shape");
+        } else if (l.ks != 3) {
+            return fail(TH_ERR_ARG, "surrogate: no ReLU conv for this shape");
+        } else if (l.cin == 16 && l.cout == 32) SUR_CONV(3, 16, 32, EPI_RELU);
+        else if (l.cin == 32 && l.cout == 32) SUR_CONV(3, 32, 32, EPI_RELU);
+        else if (l.cin == 32 && l.cout == 64) SUR_CONV(3, 32, 64, EPI_RELU);
+        else if (l.cin == 64 && l.cout == 64) SUR_CONV(3, 64, 64, EPI_RELU);
+        else if (l.cin == 64 && l.cout == 32) SUR_CONV(3, 64, 32, EPI_RELU);
+        else return fail(TH_ERR_ARG, "surrogate: no conv for this shape");
+#undef SUR_CONV
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    // dX = conv(GZ, flipped W^T): input channels = the layer's cout
+    int dgrad(int k, const bf* gz, bf* gx, int B) {
+        const LayerSpec& l = kSpec[k];
+        const long n = npix(l.level, B);
+        const int h = H >> l.level, w = W >> l.level;
+        const unsigned g = grid1(n, 64);
+#define SUR_DG(KS, CI, CO) \
+    k_conv<KS, CI, CO, EPI_LINEAR><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w)
+        if (l.ks == 1 && l.cout == 16 && l.cin == 32) SUR_DG(1, 16, 32);
+        else if (l.ks == 3 && l.cout == 32 && l.cin == 32) SUR_DG(3, 32, 32);
+        else if (l.ks == 3 && l.cout == 64 && l.cin == 32) SUR_DG(3, 64, 32);
+        else if (l.ks == 3 && l.cout == 64 && l.cin == 64) SUR_DG(3, 64, 64);
+        else if (l.ks == 3 && l.cout == 32 && l.cin == 64) SUR_DG(3, 32, 64);
+        else return fail(TH_ERR_ARG, "surrogate: no data-gradient conv for this shape");
+#undef SUR_DG
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int wgrad(int k, const bf* xin, const bf* gz, int B) {
+        const LayerSpec& l = kSpec[k];
+        const long n = npix(l.level, B);
+        const int h = H >> l.level, w = W >> l.level;
+        const int KK = l.ks * l.ks;
+        const int nch = (int)std::max(1L, std::min((long)kMaxChunks, (n + 2047) / 2048));
+        const long chunk = (n + nch - 1) / nch;
+        const dim3 g(nch, KK + 1);
+#define SUR_WG(KS, CI, CO) k_wgrad<KS, CI, CO><<<g, 64, 0, s>>>(xin, gz, wpart, n, h, w, chunk)
+        if (l.ks == 1 && l.cin == 32 && l.cout == 16) SUR_WG(1, 32, 16);
+        else if (l.ks == 3 && l.cin == 16 && l.cout == 32) SUR_WG(3, 16, 32);
+        else if (l.ks == 3 && l.cin == 32 && l.cout == 32) SUR_WG(3, 32, 32);
+        else if (l.ks == 3 && l.cin == 32 && l.cout == 64) SUR_WG(3, 32, 64);
+        else if (l.ks == 3 && l.cin == 64 && l.cout == 64) SUR_WG(3, 64, 64);
+        else if (l.ks == 3 && l.cin == 64 && l.cout == 32) SUR_WG(3, 64, 32);
+        else return fail(TH_ERR_ARG, "surrogate: no weight-gradient kernel for this shape");
+#undef SUR_WG
+        SURTRY(hipGetLastError());
+        float* gW = grad + off[k];
+        float* gb = gW + (long)l.cout * KK * l.cin;
+        k_reduce_grad<<<grid1((long)l.cout * KK * l.cin + l.cout), 256, 0, s>>>(wpart, nch, KK, l.cout, l.cin, gW, gb);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int relu_bwd(bf* g, const bf* y, long n) {
+        k_relu_bwd<<<grid1(n), 256, 0, s>>>(g, y, n);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int pool(const bf* x, bf* y, int lvl_out, int B, int C) {
+        const long n = npix(lvl_out, B) * C;
+        k_pool<<<grid1(n), 256, 0, s>>>(x, y, n, H >> lvl_out, W >> lvl_out, C);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int upadd(const bf* xc, const bf* sk, bf* y, int lvl, int B, int C) {
+        const long n = npix(lvl, B) * C;
+        k_upadd<<<grid1(n), 256, 0, s>>>(xc, sk, y, n, H >> lvl, W >> lvl, C);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int upbwd(const bf* g, bf* gc, int lvl_c, int B, int C) {
+        const long n = npix(lvl_c, B) * C;
+        k_upbwd<<<grid1(n), 256, 0, s>>>(g, gc, n, H >> lvl_c, W >> lvl_c, C);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int poolbwd_add(const bf* gp, const bf* gs, bf* g, int lvl, int B, int C) {
+        const long n = npix(lvl, B) * C;
+        k_poolbwd_add<<<grid1(n), 256, 0, s>>>(gp, gs, g, n, H >> lvl, W >> lvl, C);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+
+    int forward(int B) {
+        int rc;
+        if ((rc = conv(E0A, x0, a0, B, EPI_RELU)) || (rc = conv(E0B, a0, s0, B, EPI_RELU)) ||
+            (rc = pool(s0, p0, 1, B, 32)) || (rc = conv(E1A, p0, s1, B, EPI_RELU)) || (rc = pool(s1, p1, 2, B, 32)) ||
+            (rc = conv(E2A, p1, a2, B, EPI_RELU)) || (rc = conv(E2B, a2, s2, B, EPI_RELU)) ||
+            (rc = pool(s2, p2, 3, B, 64)) || (rc = conv(E3A, p2, s3, B, EPI_RELU)) || (rc = pool(s3, p3, 4, B, 64)) ||
+            (rc = conv(B0, p3, bb, B, EPI_RELU)) || (rc = upadd(bb, s3, u3, 3, B, 64)) ||
+            (rc = conv(D3, u3, c3, B, EPI_RELU)) || (rc = upadd(c3, s2, u2, 2, B, 64)) ||
+            (rc = conv(D2, u2, c2, B, EPI_RELU)) || (rc = upadd(c2, s1, u1, 1, B, 32)) ||
+            (rc = conv(D1, u1, c1, B, EPI_RELU)) || (rc = upadd(c1, s0, u0, 0, B, 32)) ||
+            (rc = conv(D0, u0, c0, B, EPI_RELU)) || (rc = conv(HEAD, c0, out, B, EPI_F32)))
+            return rc;
+        return TH_OK;
+    }
+
+    int backward(int B) {
+        const long n0 = npix(0, B), n1 = npix(1, B), n2 = npix(2, B), n3 = npix(3, B), n4 = npix(4, B);
+        int rc;
+        // head (linear)
+        if ((rc = wgrad(HEAD, c0, g_out, B)) || (rc = dgrad(HEAD, g_out, g_c0, B))) return rc;
+        // d0 .. d3: the input of each is up(prev) + skip
+        if ((rc = relu_bwd(g_c0, c0, n0 * 32)) || (rc = wgrad(D0, u0, g_c0, B)) || (rc = dgrad(D0, g_c0, g_u0, B)) ||
+            (rc = upbwd(g_u0, g_c1, 1, B, 32)))
+            return rc;
+        if ((rc = relu_bwd(g_c1, c1, n1 * 32)) || (rc = wgrad(D1, u1, g_c1, B)) || (rc = dgrad(D1, g_c1, g_u1, B)) ||
+            (rc = upbwd(g_u1, g_c2, 2, B, 32)))
+            return rc;
+        if ((rc = relu_bwd(g_c2, c2, n2 * 32)) || (rc = wgrad(D2, u2, g_c2, B)) || (rc = dgrad(D2, g_c2, g_u2, B)) ||
+            (rc = upbwd(g_u2, g_c3, 3, B, 64)))
+            return rc;
+        if ((rc = relu_bwd(g_c3, c3, n3 * 64)) || (rc = wgrad(D3, u3, g_c3, B)) || (rc = dgrad(D3, g_c3, g_u3, B)) ||
+            (rc = upbwd(g_u3, g_bb, 4, B, 64)))
+            return rc;
+        // bottleneck and encoder; a skip tensor's gradient adds its pool's
+        if ((rc = relu_bwd(g_bb, bb, n4 * 64)) || (rc = wgrad(B0, p3, g_bb, B)) || (rc = dgrad(B0, g_bb, g_p3, B)) ||
+            (rc = poolbwd_add(g_p3, g_u3, g_s3, 3, B, 64)))
+            return rc;
+        if ((rc = relu_bwd(g_s3, s3, n3 * 64)) || (rc = wgrad(E3A, p2, g_s3, B)) || (rc = dgrad(E3A, g_s3, g_p2, B)) ||
+            (rc = poolbwd_add(g_p2, g_u2, g_s2, 2, B, 64)))
+            return rc;
+        if ((rc = relu_bwd(g_s2, s2, n2 * 64)) || (rc = wgrad(E2B, a2, g_s2, B)) || (rc = dgrad(E2B, g_s2, g_a2, B)))
+            return rc;
+        if ((rc = relu_bwd(g_a2, a2, n2 * 64)) || (rc = wgrad(E2A, p1, g_a2, B)) || (rc = dgrad(E2A, g_a2, g_p1, B)) ||
+            (rc = poolbwd_add(g_p1, g_u1, g_s1, 1, B, 32)))
+            return rc;
+        if ((rc = relu_bwd(g_s1, s1, n1 * 32)) || (rc = wgrad(E1A, p0, g_s1, B)) || (rc = dgrad(E1A, g_s1, g_p0, B)) ||
+            (rc = poolbwd_add(g_p0, g_u0, g_s0, 0, B, 32)))
+            return rc;
+        if ((rc = relu_bwd(g_s0, s0, n0 * 32)) || (rc = wgrad(E0B, a0, g_s0, B)) || (rc = dgrad(E0B, g_s0, g_a0, B)))
+            return rc;
+        if ((rc = relu_bwd(g_a0, a0, n0 * 32)) || (rc = wgrad(E0A, x0, g_a0, B))) return rc;
+        return TH_OK;
+    }
+
+    // loss of the current outputs into *dst (device), gradient into g_out
+    int loss(int B, const float* tgt, const int* idx, double* dst) {
+        const long n0 = npix(0, B);
+        const unsigned g = grid1(n0 * 16);
+        k_mse<<<g, 256, 0, s>>>(out, tgt, idx, hw(), n0, 1.0 / ((double)n0 * kOut), g_out, lpart);
+        SURTRY(hipGetLastError());
+        k_sum_to<<<1, 256, 0, s>>>(lpart, (int)g, 1.0 / ((double)n0 * kOut), dst);
+        SURTRY(hipGetLastError());
+        return TH_OK;
+    }
+
+    int adam(float lr) {
+        ++adam_t;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)adam_t)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)adam_t)));
+        k_adam<<<grid1(nparam), 256, 0, s>>>(prm, grad, m, v, nparam, lr, b1, b2, 1e-8f, c1, c2);
+        SURTRY(hipGetLastError());
+        return pack();
+    }
+
+    // host features [B][H][W][3] -> x0 (BF16, 16 channels)
+    int upload_input(const float* in, int B) {
+        std::vector<bf> h((size_t)npix(0, B) * 16, 0);
+        for (long p = 0; p < npix(0, B); ++p)
+            for (int c = 0; c < kIn; ++c) h[p * 16 + c] = host_f2bf(in[p * kIn + c]);
+        SURTRY(hipMemcpyAsync(x0, h.data(), h.size() * sizeof(bf), hipMemcpyHostToDevice, s));
+        SURTRY(hipStreamSynchronize(s));
+        return TH_OK;
+    }
+    int download_output(float* host, int B) {
+        const long n0 = npix(0, B);
+        std::vector<float> h((size_t)n0 * 16);
+        SURTRY(hipMemcpyAsync(h.data(), out, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        SURTRY(hipStreamSynchronize(s));
+        for (long p = 0; p < n0; ++p)
+            for (int c = 0; c < kOut; ++c) host[p * kOut + c] = h[p * 16 + c];
+        return TH_OK;
+    }
+};
+
+}  // namespace sur
+}  // namespace th
+
+using namespace th;
+using namespace th::sur;
+
+struct th_surrogate {
+    Net net;
+};
+
+extern "C" {
+
+long th_surrogate_param_count(void) { return total_params(); }
+
+int th_surrogate_create(int H, int W, unsigned long long seed, th_surrogate** out) {
+    clear_error();
+    if (!out) return fail(TH_ERR_ARG, "null out");
+    if (H < 16 || W < 16 || H % 16 || W % 16) return fail(TH_ERR_ARG, "surrogate grid must be a multiple of 16");
+    th_surrogate* s = new th_surrogate;
+    int rc = s->net.init(H, W, seed);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return TH_OK;
+}
+
+void th_surrogate_destroy(th_surrogate* s) { delete s; }
+
+#define TH_SUR(s) \
+    if (!(s)) return fail(TH_ERR_ARG, "null surrogate"); \
+    clear_error();
+
+int th_surrogate_get_params(th_surrogate* s, float* out) {
+    TH_SUR(s);
+    if (!out) return fail(TH_ERR_ARG, "null out");
+    SURTRY(hipMemcpyAsync(out, s->net.prm, s->net.nparam * sizeof(float), hipMemcpyDeviceToHost, s->net.s));
+    SURTRY(hipStreamSynchronize(s->net.s));
+    return TH_OK;
+}
+
+int th_surrogate_set_params(th_surrogate* s, const float* in) {
+    TH_SUR(s);
+    if (!in) return fail(TH_ERR_ARG, "null params");
+    return s->net.set_params(in);
+}
+
+int th_surrogate_predict(th_surrogate* s, const float* in, int batch, float* out) {
+    TH_SUR(s);
+    if (!in || !out || batch < 1) return fail(TH_ERR_ARG, "bad argument");
+    Net& n = s->net;
+    int rc;
+    if ((rc = n.ensure_batch(batch)) || (rc = n.upload_input(in, batch)) || (rc = n.forward(batch))) return rc;
+    return n.download_output(out, batch);
+}
+
+int th_surrogate_predict_device(th_surrogate* s, const float* in, const float** d_out) {
+    TH_SUR(s);
+    if (!in || !d_out) return fail(TH_ERR_ARG, "bad argument");
+    Net& n = s->net;
+    int rc;
+    if ((rc = n.ensure_batch(1)) || (rc = n.upload_input(in, 1)) || (rc = n.forward(1))) return rc;
+    SURTRY(hipStreamSynchronize(n.s));
+    *d_out = n.out;
+    return TH_OK;
+}
+
+int th_surrogate_grad(th_surrogate* s, const float* in, const float* tgt, int batch, double* loss, float* grad) {
+    TH_SUR(s);
+    if (!in || !tgt || batch < 1) return fail(TH_ERR_ARG, "bad argument");
+    Net& n = s->net;
+    int rc;
+    if ((rc = n.ensure_batch(batch)) || (rc = n.upload_input(in, batch))) return rc;
+    SURTRY(hipMemcpyAsync(n.tgt_b, tgt, (size_t)n.npix(0, batch) * kOut * sizeof(float), hipMemcpyHostToDevice, n.s));
+    double* dsum = nullptr;
+    SURTRY(hipMalloc(&dsum, sizeof(double)));
+    rc = n.forward(batch);
+    if (!rc) rc = n.loss(batch, n.tgt_b, nullptr, dsum);
+    if (!rc) rc = n.backward(batch);
+    double h = 0.0;
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpyAsync(&h, dsum, sizeof(double), hipMemcpyDeviceToHost, n.s);
+        if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, n.grad, n.nparam * sizeof(float), hipMemcpyDeviceToHost, n.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(n.s);
+    }
+    (void)hipFree(dsum);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("surrogate grad: ") + hipGetErrorString(e));
+    if (loss) *loss = h;
+    return TH_OK;
+}
+
+int th_surrogate_dataset_reserve(th_surrogate* s, int capacity) {
+    TH_SUR(s);
+    Net& n = s->net;
+    if (capacity < 1) return fail(TH_ERR_ARG, "capacity must be >= 1");
+    if (n.ds_cap) return fail(TH_ERR_ARG, "dataset already reserved");
+    int rc;
+    if ((rc = n.dalloc(&n.ds_in, (size_t)capacity * n.hw() * kInStore, n.own)) ||
+        (rc = n.dalloc(&n.ds_tgt, (size_t)capacity * n.hw() * kOut, n.own)))
+        return rc;
+    n.ds_cap = capacity;
+    n.ds_n = 0;
+    return TH_OK;
+}
+
+int th_surrogate_dataset_add(th_surrogate* s, const float* in, const float* tgt, int count) {
+    TH_SUR(s);
+    Net& n = s->net;
+    if (!in || !tgt || count < 1) return fail(TH_ERR_ARG, "bad argument");
+    if (n.ds_n + count > n.ds_cap) return fail(TH_ERR_ARG, "dataset capacity exceeded");
+    const long hw = n.hw();
+    std::vector<bf> h((size_t)count * hw * kInStore, 0);
+    for (long p = 0; p < (long)count * hw; ++p)
+        for (int c = 0; c < kIn; ++c) h[p * kInStore + c] = host_f2bf(in[p * kIn + c]);
+    SURTRY(hipMemcpyAsync(n.ds_in + (size_t)n.ds_n * hw * kInStore, h.data(), h.size() * sizeof(bf),
+                          hipMemcpyHostToDevice, n.s));
+    SURTRY(hipMemcpyAsync(n.ds_tgt + (size_t)n.ds_n * hw * kOut, tgt, (size_t)count * hw * kOut * sizeof(float),
+                          hipMemcpyHostToDevice, n.s));
+    SURTRY(hipStreamSynchronize(n.s));
+    n.ds_n += count;
+    return TH_OK;
+}
+
+int th_surrogate_dataset_size(th_surrogate* s) { return s ? s->net.ds_n : -1; }
+
+// Adam on random batches of samples [0, n_train): `steps` steps, learning
+// rate decaying as a cosine from lr to lr_final; per-step losses to host.
+int th_surrogate_train(th_surrogate* s, int steps, int batch, float lr, float lr_final, unsigned long long seed,
+                       int n_train, float* losses, double* ms) {
+    TH_SUR(s);
+    Net& n = s->net;
+    if (steps < 1 || batch < 1 || n_train < batch || n_train > n.ds_n)
+        return fail(TH_ERR_ARG, "bad training arguments (n_train must be in [batch, dataset size])");
+    int rc;
+    if ((rc = n.ensure_batch(batch))) return rc;
+    if (n.dloss_cap < steps) {
+        if (n.dloss) (void)hipFree(n.dloss);
+        n.dloss = nullptr;
+        SURTRY(hipMalloc(&n.dloss, steps * sizeof(double)));
+        n.dloss_cap = steps;
+    }
+    SplitMix rng{seed};
+    std::vector<int> order(n_train);
+    std::iota(order.begin(), order.end(), 0);
+    size_t pos = order.size();
+    std::vector<int> idx(batch);
+    int* pinned = nullptr;
+    SURTRY(hipHostMalloc((void**)&pinned, (size_t)steps * batch * sizeof(int), hipHostMallocDefault));
+    for (int t = 0; t < steps; ++t)
+        for (int b = 0; b < batch; ++b) {
+            if (pos >= order.size()) {  // new epoch: Fisher-Yates shuffle
+                for (size_t i = order.size() - 1; i > 0; --i) std::swap(order[i], order[rng.next() % (i + 1)]);
+                pos = 0;
+            }
+            pinned[(size_t)t * batch + b] = order[pos++];
+        }
+    int* didx = nullptr;
+    hipError_t e = hipMalloc(&didx, (size_t)steps * batch * sizeof(int));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(didx, pinned, (size_t)steps * batch * sizeof(int), hipMemcpyHostToDevice, n.s);
+    if (e != hipSuccess) {
+        (void)hipHostFree(pinned);
+        if (didx) (void)hipFree(didx);
+        return fail(TH_ERR_HIP, std::string("surrogate train: ") + hipGetErrorString(e));
+    }
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    const long n0 = n.npix(0, batch);
+    for (int t = 0; t < steps && !rc; ++t) {
+        const int* bi = didx + (size_t)t * batch;
+        k_gather<<<grid1(n0 * 16), 256, 0, n.s>>>(n.ds_in, bi, n.x0, n0 * 16, n.hw());
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        const double frac = steps > 1 ? (double)t / (steps - 1) : 1.0;
+        const float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(3.141592653589793 * frac)));
+        if ((rc = n.forward(batch)) || (rc = n.loss(batch, n.ds_tgt, bi, n.dloss + t)) || (rc = n.backward(batch)) ||
+            (rc = n.adam(lr_t)))
+            break;
+    }
+    std::vector<double> h(steps, 0.0);
+    if (!rc && e == hipSuccess) e = hipMemcpyAsync(h.data(), n.dloss, steps * sizeof(double), hipMemcpyDeviceToHost, n.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(n.s);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    (void)hipFree(didx);
+    (void)hipHostFree(pinned);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("surrogate train: ") + hipGetErrorString(e));
+    if (losses)
+        for (int t = 0; t < steps; ++t) losses[t] = (float)h[t];
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return TH_OK;
+}
+
+// Mean squared error (scaled units) over dataset samples [first, first+count).
+int th_surrogate_eval(th_surrogate* s, int first, int count, double* mse) {
+    TH_SUR(s);
+    Net& n = s->net;
+    if (!mse || first < 0 || count < 1 || first + count > n.ds_n) return fail(TH_ERR_ARG, "bad range");
+    int rc;
+    const int B = std::min(count, std::max(1, n.Bcap > 0 ? n.Bcap : 8));
+    if ((rc = n.ensure_batch(B))) return rc;
+    double* dsum = nullptr;
+    SURTRY(hipMalloc(&dsum, sizeof(double) * ((count + B - 1) / B)));
+    std::vector<int> idx(B);
+    int* pinned = nullptr;
+    hipError_t e = hipHostMalloc((void**)&pinned, (size_t)count * sizeof(int), hipHostMallocDefault);
+    double total = 0.0;
+    long npx = 0;
+    int nb = 0;
+    std::vector<int> sizes;
+    if (e == hipSuccess) {
+        for (int i = 0; i < count; ++i) pinned[i] = first + i;
+        int* didx = nullptr;
+        e = hipMalloc(&didx, (size_t)count * sizeof(int));
+        if (e == hipSuccess) e = hipMemcpyAsync(didx, pinned, (size_t)count * sizeof(int), hipMemcpyHostToDevice, n.s);
+        for (int b0 = 0; b0 < count && e == hipSuccess && !rc; b0 += B, ++nb) {
+            const int bs = std::min(B, count - b0);
+            const long n0 = n.npix(0, bs);
+            k_gather<<<grid1(n0 * 16), 256, 0, n.s>>>(n.ds_in, didx + b0, n.x0, n0 * 16, n.hw());
+            e = hipGetLastError();
+            if (e == hipSuccess && !(rc = n.forward(bs))) rc = n.loss(bs, n.ds_tgt, didx + b0, dsum + nb);
+            sizes.push_back(bs);
+        }
+        std::vector<double> h(nb);
+        if (e == hipSuccess && !rc) e = hipMemcpyAsync(h.data(), dsum, nb * sizeof(double), hipMemcpyDeviceToHost, n.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(n.s);
+        for (int k = 0; k < nb; ++k) {
+            total += h[k] * sizes[k];
+            npx += sizes[k];
+        }
+        if (didx) (void)hipFree(didx);
+    }
+    if (pinned) (void)hipHostFree(pinned);
+    (void)hipFree(dsum);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("surrogate eval: ") + hipGetErrorString(e));
+    *mse = total / npx;
+    return TH_OK;
+}
+
+// Average time of one forward pass (batch B, inputs as they are) and its
+// MFMA work: 2 H W cin cout k^2 per layer and image.
+int th_surrogate_time_forward(th_surrogate* s, int batch, int reps, double* avg_ms, double* flops) {
+    TH_SUR(s);
+    Net& n = s->net;
+    if (batch < 1 || reps < 1 || !avg_ms || !flops) return fail(TH_ERR_ARG, "bad argument");
+    int rc;
+    if ((rc = n.ensure_batch(batch)) || (rc = n.forward(batch))) return rc;
+    hipEvent_t e0, e1;
+    SURTRY(hipEventCreate(&e0));
+    SURTRY(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, n.s);
+    for (int r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = n.forward(batch);
+    if (!rc && e == hipSuccess) e = hipEventRecord(e1, n.s);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("surrogate timing: ") + hipGetErrorString(e));
+    double f = 0.0;
+    for (const LayerSpec& l : kSpec) f += 2.0 * n.npix(l.level, batch) * l.cin * l.cout * l.ks * l.ks;
+    *avg_ms = ms / reps;
+    *flops = f;
+    return TH_OK;
+}
+
+}  // extern "C"

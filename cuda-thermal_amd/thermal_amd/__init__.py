@@ -127,6 +127,23 @@ SIGNATURES = {
     "th_transient_default_opts": (None, [C.POINTER(TransientOpts)]),
     "th_run_transient": (C.c_int, [_vp, C.POINTER(TransientOpts), C.POINTER(StepRecord), C.c_int, _ip, _dp,
                                    _ip, _dp]),
+    "th_surrogate_param_count": (C.c_long, []),
+    "th_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
+    "th_surrogate_destroy": (None, [_vp]),
+    "th_surrogate_get_params": (C.c_int, [_vp, _fp]),
+    "th_surrogate_set_params": (C.c_int, [_vp, _fp]),
+    "th_surrogate_predict": (C.c_int, [_vp, _fp, C.c_int, _fp]),
+    "th_surrogate_predict_device": (C.c_int, [_vp, _fp, C.POINTER(_fp)]),
+    "th_surrogate_grad": (C.c_int, [_vp, _fp, _fp, C.c_int, _dp, _fp]),
+    "th_surrogate_dataset_reserve": (C.c_int, [_vp, C.c_int]),
+    "th_surrogate_dataset_add": (C.c_int, [_vp, _fp, _fp, C.c_int]),
+    "th_surrogate_dataset_size": (C.c_int, [_vp]),
+    "th_surrogate_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, C.c_int, _fp,
+                                     _dp]),
+    "th_surrogate_eval": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "th_surrogate_time_forward": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "th_solver_set_x_lifted": (C.c_int, [_vp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                         C.c_int, _ip, _dp, _dp]),
 }
 
 _lib = None
@@ -414,6 +431,17 @@ class Solver:
         _check(lib().th_solver_apply_operator(self.h, _d(x), _d(y)))
         return y
 
+    def set_x_lifted(self, d_field, Hc, Wc, C_, zc_coarse, zc_fine, scale=1.0, passes=2):
+        """Initial guess lifted from a coarse device field (th_solver_set_x_lifted),
+        with the plain warm start as the fallback. Returns (used, relres_plain,
+        relres_lift)."""
+        zc = np.ascontiguousarray(zc_coarse, np.float64)
+        zf = np.ascontiguousarray(zc_fine, np.float64)
+        used, rp, rl = C.c_int(), C.c_double(), C.c_double()
+        _check(lib().th_solver_set_x_lifted(self.h, C.cast(d_field, _fp), Hc, Wc, C_, len(zc), _d(zc), _d(zf),
+                                            scale, passes, C.byref(used), C.byref(rp), C.byref(rl)))
+        return bool(used.value), rp.value, rl.value
+
 
 def steady_solve(model, tol=1e-10, maxit=50000, **kw):
     """gpu_pcg (pcg3d_gpu.cu:686-719): steady solve from T = 0, timed."""
@@ -448,3 +476,88 @@ def run_transient(solver, dt0=1e-5, growth=1.1, dt_max=0.05, t_end=1.0, tol=1e-9
                 q_out=np.array([recs[i].q_out for i in range(k)]),
                 q_stored=np.array([recs[i].q_stored for i in range(k)]), T=T, converged=bool(conv.value),
                 solve_ms=ms.value)
+
+
+class Surrogate:
+    """The learned warm start (th_surrogate_*): a U-Net on an H x W grid from
+    3 features per cell to 10 plane temperature rises (units of 50 K)."""
+
+    N_IN, N_OUT, T_SCALE = 3, 10, 50.0
+
+    def __init__(self, H=256, W=256, seed=1):
+        self.H, self.W = H, W
+        h = _vp()
+        _check(lib().th_surrogate_create(H, W, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().th_surrogate_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @staticmethod
+    def param_count():
+        return lib().th_surrogate_param_count()
+
+    def get_params(self):
+        p = np.empty(self.param_count(), np.float32)
+        _check(lib().th_surrogate_get_params(self.h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.param_count():
+            raise ValueError("wrong parameter count")
+        _check(lib().th_surrogate_set_params(self.h, _f(p)))
+
+    def predict(self, x):
+        """x [B,H,W,3] float -> [B,H,W,10] float (units of T_SCALE)."""
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty(x.shape[:3] + (self.N_OUT,), np.float32)
+        _check(lib().th_surrogate_predict(self.h, _f(x), x.shape[0], _f(out)))
+        return out
+
+    def predict_device(self, x):
+        """One image; device pointer to the FP32 [H][W][16] output."""
+        x = np.ascontiguousarray(x, np.float32)
+        p = _fp()
+        _check(lib().th_surrogate_predict_device(self.h, _f(x), C.byref(p)))
+        return C.cast(p, C.c_void_p).value
+
+    def grad(self, x, tgt):
+        x = np.ascontiguousarray(x, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        loss = C.c_double()
+        g = np.empty(self.param_count(), np.float32)
+        _check(lib().th_surrogate_grad(self.h, _f(x), _f(tgt), x.shape[0], C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def reserve(self, capacity):
+        _check(lib().th_surrogate_dataset_reserve(self.h, capacity))
+
+    def add(self, x, tgt):
+        x = np.ascontiguousarray(x, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        _check(lib().th_surrogate_dataset_add(self.h, _f(x), _f(tgt), x.shape[0]))
+
+    def size(self):
+        return lib().th_surrogate_dataset_size(self.h)
+
+    def train(self, steps, batch=16, lr=2e-3, lr_final=1e-4, seed=7, n_train=None):
+        losses = np.empty(steps, np.float32)
+        ms = C.c_double()
+        _check(lib().th_surrogate_train(self.h, steps, batch, lr, lr_final, seed, n_train or self.size(),
+                                        _f(losses), C.byref(ms)))
+        return losses, ms.value
+
+    def eval(self, first, count):
+        v = C.c_double()
+        _check(lib().th_surrogate_eval(self.h, first, count, C.byref(v)))
+        return v.value
+
+    def time_forward(self, batch=1, reps=20):
+        ms, fl = C.c_double(), C.c_double()
+        _check(lib().th_surrogate_time_forward(self.h, batch, reps, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value

@@ -203,6 +203,54 @@ TH_API int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_re
                             int rec_cap, int* nsteps, double* T_final, int* all_converged,
                             double* solve_ms);
 
+/* ---- learned warm start (no reference counterpart) ------------------------
+ * A U-Net surrogate (BF16 MFMA kernels, FP32 accumulation) mapping, on an
+ * H x W grid over the die, 3 input features per cell (power density of die
+ * 0 and die 1 in MW/m^2, TSV copper fraction) to 10 outputs per cell (the
+ * temperature rise of the 10 planes of the two-tier stack, in units of
+ * 50 K). Feature and output arrays are host float arrays [B][H][W][C].
+ * Parameters are one flat float vector (per layer W[cout][tap][cin], then
+ * bias[cout]). One handle owns one HIP stream and is not thread-safe. */
+typedef struct th_surrogate th_surrogate;
+TH_API long th_surrogate_param_count(void);
+TH_API int th_surrogate_create(int H, int W, unsigned long long seed, th_surrogate** out);
+TH_API void th_surrogate_destroy(th_surrogate* s);
+TH_API int th_surrogate_get_params(th_surrogate* s, float* out);
+TH_API int th_surrogate_set_params(th_surrogate* s, const float* in);
+TH_API int th_surrogate_predict(th_surrogate* s, const float* in, int batch, float* out);
+/* One image; the output stays on the device: float [H][W][16], the first 10
+ * channels used, valid until the next call on s. */
+TH_API int th_surrogate_predict_device(th_surrogate* s, const float* in, const float** d_out);
+/* Loss (mean squared error over the 10 outputs) and its gradient with
+ * respect to the flat parameters (grad may be NULL). */
+TH_API int th_surrogate_grad(th_surrogate* s, const float* in, const float* tgt, int batch, double* loss,
+                             float* grad);
+/* Training set held in device memory: features in BF16, targets in FP32. */
+TH_API int th_surrogate_dataset_reserve(th_surrogate* s, int capacity);
+TH_API int th_surrogate_dataset_add(th_surrogate* s, const float* in, const float* tgt, int count);
+TH_API int th_surrogate_dataset_size(th_surrogate* s);
+/* Adam on random batches of samples [0, n_train), learning rate decaying
+ * from lr to lr_final (cosine); per-step losses (steps floats) to host. */
+TH_API int th_surrogate_train(th_surrogate* s, int steps, int batch, float lr, float lr_final,
+                              unsigned long long seed, int n_train, float* losses, double* ms);
+TH_API int th_surrogate_eval(th_surrogate* s, int first, int count, double* mse);
+/* Average forward time of a batch and its MFMA work (flop per forward). */
+TH_API int th_surrogate_time_forward(th_surrogate* s, int batch, int reps, double* avg_ms, double* flops);
+
+/* Initial guess from a coarse field (for instance the surrogate's output):
+ * field is a device float array [Hc][Wc][C] whose channels 0..nzc-1 are
+ * planes with centres zc_coarse (m); each of the solver's planes (centres
+ * zc_fine, nz entries) is interpolated linearly in z and bilinearly over
+ * the die, times `scale`, then `passes` red-black column line relaxations
+ * of the steady operator (each a tridiagonal solve per column) follow.
+ * Fallback: if ||P - A x|| of the result is not below that of the current
+ * x (the plain warm start), x is left as it was. *used = 1 if the lifted
+ * guess was kept; relres_plain / relres_lift receive both starting
+ * residuals ||P - A x|| / ||P||. */
+TH_API int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc,
+                                  const double* zc_coarse, const double* zc_fine, double scale, int passes,
+                                  int* used, double* relres_plain, double* relres_lift);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,157 @@
+"""GPU tests of the learned warm start: the surrogate's BF16 MFMA forward
+and backward kernels against the CPU oracle (oracle/surrogate.py), training
+on solver-generated data, and the lifted initial guess with its fallback."""
+import os
+
+import numpy as np
+import pytest
+
+import surrogate as S  # oracle/surrogate.py
+import thermal_amd as T
+from thermal_amd import surrogate as TS
+from thermal_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+class DeviceArray:
+    """A float32 array copied to device memory through the HIP runtime the
+    library itself links (test plumbing)."""
+
+    def __init__(self, a):
+        import ctypes as C
+
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        a = np.ascontiguousarray(a, np.float32)
+        p = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(p), C.c_size_t(a.nbytes)) == 0
+        assert self.hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0  # host to device
+        self.ptr = p.value
+
+    def __del__(self):
+        self.hip.hipFree(self.C.c_void_p(self.ptr))
+
+
+def features_batch(B, n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((B, n, n, 3), np.float32)
+    x[..., :2] = rng.gamma(1.0, 0.3, (B, n, n, 2))
+    x[..., 2] = rng.random((B, n, n)) < 0.1
+    return x
+
+
+def padded(x):
+    xp = np.zeros(x.shape[:3] + (16,), np.float32)
+    xp[..., :3] = x
+    return xp
+
+
+@pytest.mark.parametrize("n,B", [(32, 2), (64, 1)])
+def test_forward_matches_oracle(n, B):
+    sur = T.Surrogate(n, n, seed=11)
+    p = sur.get_params()
+    x = features_batch(B, n, 1)
+    out = sur.predict(x)
+    ref, _ = S.forward(p, padded(x), round_bf16=True)
+    ref = ref[..., :S.N_OUT]
+    scale = np.abs(ref).max()
+    assert scale > 0
+    # FP32 accumulation order differs; BF16 storage then rounds the same
+    # values, except for rare ties that propagate as ~1 bf16 ulp.
+    assert np.abs(out - ref).max() < 2e-2 * scale, np.abs(out - ref).max() / scale
+    assert np.sqrt(np.mean((out - ref) ** 2)) < 3e-3 * scale
+
+
+def test_gradients_match_oracle():
+    n, B = 32, 2
+    sur = T.Surrogate(n, n, seed=12)
+    p = sur.get_params()
+    x = features_batch(B, n, 2)
+    tgt = np.random.default_rng(3).random((B, n, n, S.N_OUT)).astype(np.float32)
+    loss, g = sur.grad(x, tgt)
+    ref_loss, ref_g = S.loss_and_grad(p, padded(x), tgt, round_bf16=True)
+    assert abs(loss - ref_loss) < 1e-3 * ref_loss
+    off = 0
+    for name, _, ci, co, ks, _ in S.LAYERS:
+        k = co * ks * ks * ci + co
+        a, b = g[off:off + k].astype(np.float64), ref_g[off:off + k]
+        off += k
+        nb = np.linalg.norm(b)
+        assert nb > 0, name
+        # BF16 gradients between layers: a few parts in a hundred per layer
+        assert np.linalg.norm(a - b) < 5e-2 * nb, (name, np.linalg.norm(a - b) / nb)
+        assert a @ b > 0.998 * np.linalg.norm(a) * nb, name
+
+
+def test_training_reduces_loss_on_solver_data():
+    """A few hundred Adam steps on 64x64 solver-generated samples: the
+    training loss falls and the held-out error beats the untrained net."""
+    n = 64
+    sur = T.Surrogate(n, n, seed=3)
+    sur.reserve(72)
+    info = TS.generate(sur, 72, n_tsv=3, seed0=500, log=None)
+    assert info["samples"] == 72
+    before = sur.eval(64, 8)
+    losses, ms = sur.train(300, batch=8, lr=3e-3, lr_final=3e-4, seed=5, n_train=64)
+    after = sur.eval(64, 8)
+    assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5])
+    assert after < 0.2 * before, (before, after)
+
+
+def test_lift_of_exact_field_and_fallback():
+    """The exact solution given as a coarse field on the solver's own grid
+    lifts to itself (a converged start); a zero field loses to the plain
+    warm start and leaves x untouched."""
+    n = 64
+    spec = TS.sample_spec(9, 7000)
+    m = W.build(spec, n)
+    s = T.Solver(m, precision="fp64")
+    s.set_x_zero()
+    s.set_rhs_steady()
+    s.solve(1e-12, 200)
+    x = s.get_x()
+    field = np.zeros((n, n, 16), np.float32)
+    field[..., :10] = np.moveaxis(x.reshape(10, n, n), 0, -1)
+    d = DeviceArray(field)
+    zc = TS.plane_centres(W.config0_layers())
+    s2 = T.Solver(m, precision="fp64")
+    s2.set_x_zero()
+    used, rp, rl = s2.set_x_lifted(d.ptr, n, n, 16, zc, zc, scale=1.0, passes=0)
+    assert used and np.isclose(rp, 1.0) and rl < 1e-2
+    s2.set_rhs_steady()
+    st = s2.solve(1e-6, 50)
+    assert st["converged"] and st["iters"] <= 4
+    z = DeviceArray(np.zeros_like(field))
+    s3 = T.Solver(m, precision="fp64")
+    s3.set_x(x)
+    used, rp, rl = s3.set_x_lifted(z.ptr, n, n, 16, zc, zc, scale=1.0, passes=1)
+    assert not used and rp < 1e-10
+    assert np.array_equal(s3.get_x(), x)
+
+
+@pytest.mark.skipif(not os.path.exists(TS.WEIGHTS), reason="no trained weights")
+def test_trained_surrogate_error_and_warm_start():
+    """The trained network on floorplans it has not seen (256 x 256 x 10):
+    its error against the converged PCG answer, and iterations to 1e-10
+    from T = 0 and from the surrogate's lifted guess (with fallback)."""
+    sur = TS.load()
+    fine = W.config0_layers()
+    errs, cold, warm = [], [], []
+    for k in range(4):
+        spec = TS.sample_spec(990000 + k, 7000 + 2 * (60 + k))
+        m = W.build(spec, 256)
+        s = T.Solver(m)
+        s.set_x_zero()
+        s.set_rhs_steady()
+        cold.append(s.solve(1e-10, 200)["iters"])
+        xt = s.get_x().reshape(10, 256, 256)
+        pred = sur.predict(TS.features(spec)[None])[0] * T.Surrogate.T_SCALE
+        errs.append(np.abs(np.moveaxis(pred, -1, 0) - xt).max() / xt.max())
+        s.set_x_zero()
+        used, rp, rl, _ = TS.warm_start(s, sur, spec, fine)
+        s.set_rhs_steady()
+        warm.append(s.solve(1e-10, 200)["iters"])
+    print("surrogate max rel error", errs, "cold", cold, "warm", warm)
+    assert max(errs) < 0.1
+    assert np.mean(warm) <= np.mean(cold)
