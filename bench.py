@@ -128,6 +128,86 @@ def transient_leg():
     return out
 
 
+def learned_leg(model, spec, steps, warmup, x_ref):
+    """Mixed solve warm-started by the surrogate: features of the floorplan
+    at 256 x 256, forward pass, lift to 1024 x 1024 x 16 (bilinear, linear in
+    z, one red-black column line relaxation), fallback to T = 0 if the
+    lifted guess starts with the larger residual, then the solve. Timed from
+    the forward pass to the converged answer."""
+    from thermal_amd import surrogate as TS
+
+    if not os.path.exists(TS.WEIGHTS):
+        return None
+    sur = TS.load()
+    s = T.Solver(model, true_residual=True)
+    fine = W.config2_layers()
+    feats = TS.features(spec, TS.GRID)  # input preparation (from the case), not timed
+    ms, st, info = [], None, None
+    for k in range(warmup + steps):
+        s.set_x_zero()
+        t0 = time.perf_counter()
+        d = sur.predict_device(feats)
+        used, rp, rl = s.set_x_lifted(d, TS.GRID, TS.GRID, 16, TS.COARSE_Z, TS.plane_centres(fine),
+                                      scale=T.Surrogate.T_SCALE, passes=1)
+        s.set_rhs_steady()
+        st = s.solve(TOL, 500)
+        if k >= warmup:
+            ms.append((time.perf_counter() - t0) * 1e3)
+        info = dict(used=used, relres_start_plain=rp, relres_start_lift=rl)
+    x = s.get_x()
+    P = model.total_power()
+    fwd_ms, fwd_flop = sur.time_forward(1, 20)
+    res = dict(ms=float(np.mean(ms)), ms_min=float(np.min(ms)), iters=st["iters"], converged=st["converged"],
+               true_relres=s.true_relres(), energy_balance=abs(s.heat_out() - P) / P,
+               max_abs_err_vs_fp64_K=float(np.abs(x - x_ref).max()), forward_ms=fwd_ms,
+               forward_tflops=fwd_flop / (fwd_ms * 1e-3) / 1e12, **info)
+    log(f"  mixed+learned: {res['ms']:.2f} ms/solve, {res['iters']} iterations, lifted guess used: {used} "
+        f"(start rel-res {rl:.2e} lifted vs {rp:.2e} plain), forward {fwd_ms:.3f} ms")
+    s.close()
+    sur.close()
+    return res
+
+
+def batched_leg(count=64):
+    """The batched workload: `count` steady solves of stacks of 128^2, 256^2
+    or 512^2 x 10 cells (seeded), blocks switched on with p = 0.6; one
+    batched solve (autotuned shapes) against the same solves one after
+    another with the single-instance solver."""
+    inst = W.config3_instances(count=count)
+    t0 = time.perf_counter()
+    models = [W.build(spec, n) for n, spec in inst]
+    cells = sum(m.n for m in models)
+    log(f"  batched: {count} instances, {cells / 1e6:.1f} M cells, models built in {time.perf_counter() - t0:.1f} s")
+    b = T.Batch(models)
+    tune = b.autotune(3)
+    b.solve(TOL, 500)  # warm-up
+    r = b.solve(TOL, 500)
+    its = r["iters"]
+    seq_ms, seq_its = 0.0, []
+    for m in models:
+        s = T.Solver(m)
+        s.set_x_zero()
+        s.set_rhs_steady()
+        s.solve(TOL, 500)  # warm-up
+        s.set_x_zero()
+        t1 = time.perf_counter()
+        s.set_rhs_steady()
+        st = s.solve(TOL, 500)
+        seq_ms += (time.perf_counter() - t1) * 1e3
+        seq_its.append(st["iters"])
+        s.close()
+    out = dict(instances=count, cells=cells, batch_ms=round(r["ms"], 2), solves_per_s=round(count / r["ms"] * 1e3, 1),
+               sequential_ms=round(seq_ms, 2), sequential_solves_per_s=round(count / seq_ms * 1e3, 1),
+               speedup=round(seq_ms / r["ms"], 2), all_converged=bool(r["converged"].all()),
+               iters_p50=float(np.percentile(its, 50)), iters_p99=float(np.percentile(its, 99)),
+               sequential_iters_p50=float(np.percentile(seq_its, 50)),
+               sequential_iters_p99=float(np.percentile(seq_its, 99)), autotune=tune,
+               workload="64 steady solves, n in {128,256,512} x 10, 12+12 blocks on with p=0.6, TSV clusters")
+    log(f"  batched: {out}")
+    b.close()
+    return out
+
+
 def kernels_only(n):
     """Sweep and copy kernels only (for counter runs under rocprofv3 --pmc)."""
     folder, m = make_case(n, 1000)
@@ -150,6 +230,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single solve "
@@ -186,6 +268,8 @@ def main():
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
     tr = None if args.no_transient else transient_leg()
+    learned = None if args.no_learned else learned_leg(model, W.config2_spec(args.seed), args.steps, args.warmup, x64)
+    batched = None if args.no_batched else batched_leg()
 
     cpu = None
     if not args.no_cpu:
@@ -193,23 +277,30 @@ def main():
         cpu = cpu_baseline(folder, n, r64["iters"])
         log(f"  cpu: {cpu}")
 
+    head = learned if learned else rmx
+    head_mode = ("mixed+learned (surrogate warm start, FP32 V-cycle, FP64 flexible PCG)" if learned
+                 else "mixed (FP32 V-cycle, FP64 flexible PCG)")
+    modes = {"fp64": r64, "mixed": rmx}
+    if learned:
+        modes["mixed+learned"] = learned
     line = {
         "metric": METRIC,
-        "value": round(rmx["ms"], 3),
+        "value": round(head["ms"], 3),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(rmx["ms"], 3),
+        "ms_per_step": round(head["ms"], 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 PCG + f32 V-cycle",
         "data": "synthetic (seeded floorplan and TSV clusters)",
         "config": {"workload": f"{n}x{n}x16 TSV stack, steady MG-PCG from T=0 to true rel-res 1e-10",
-                   "seed": args.seed, "cells": model.n, "mode": "mixed (FP32 V-cycle, FP64 flexible PCG)"},
-        "modes": {"fp64": r64, "mixed": rmx},
-        "speedup_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+                   "seed": args.seed, "cells": model.n, "mode": head_mode},
+        "modes": modes,
+        "speedup_vs_fp64": round(r64["ms"] / head["ms"], 3),
+        "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
         "max_abs_err_mixed_vs_fp64_K": err_mixed,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
@@ -218,6 +309,7 @@ def main():
         "kernels_GBps": {"copy": round(copy_bytes / copy_ms / 1e6, 1), "apply": round(apply_bytes / apply_ms / 1e6, 1),
                          "sweep": round(achieved, 1)},
         "transient": tr,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -77,6 +77,18 @@ struct State {
     StepRec last;
 };
 
+// Block coordinates of a kernel body: the hardware ones (hw_blk), or, in a
+// batched launch over many instances, those of the block within its
+// instance's own grid.
+struct Blk {
+    int x, y, z, gx, gy, gz;
+    __device__ int flat() const { return x + gx * (y + gy * z); }
+    __device__ bool first() const { return x == 0 && y == 0 && z == 0; }
+};
+__device__ __forceinline__ Blk hw_blk() {
+    return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+
 // Conductance arrays of one level (split layout).
 template <class T>
 struct Ops {
@@ -99,12 +111,11 @@ __device__ __forceinline__ double wave_max(double v) {
 // Block totals of NV values into part[block * NV + k] (bit k of KMAX set:
 // max instead of sum). Every thread of the block must call it.
 template <int NV, unsigned KMAX = 0>
-__device__ void block_partials(double (&v)[NV], double* __restrict__ part) {
+__device__ void block_partials(double (&v)[NV], double* __restrict__ part, int bid) {
     __shared__ double sh[NV][16];
     const int nthr = blockDim.x * blockDim.y;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = tid & 63, w = tid >> 6, nw = (nthr + 63) >> 6;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((KMAX >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
     if (lane == 0)
@@ -116,6 +127,10 @@ __device__ void block_partials(double (&v)[NV], double* __restrict__ part) {
         for (int q = 1; q < nw; ++q) s = ((KMAX >> tid) & 1) ? fmax(s, sh[tid][q]) : s + sh[tid][q];
         part[(size_t)bid * NV + tid] = s;
     }
+}
+template <int NV, unsigned KMAX = 0>
+__device__ void block_partials(double (&v)[NV], double* __restrict__ part) {
+    block_partials<NV, KMAX>(v, part, (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
 }
 
 // Grid totals from nblk block partials, in block order (one block of
@@ -184,8 +199,9 @@ __device__ void block_sum_partials(const double* __restrict__ part, int nblk, do
 // decision; block 0 records it. Returns true when the solve is finished
 // (pcg3d_cpu.cpp:141-176: b = 0, ||r||/||b|| < tol, or maxit reached).
 enum { CONS_NONE = 0, CONS_START = 1, CONS_ITER = 2 };
-__device__ inline bool consume_convergence(const double* __restrict__ part, int nblk, int mode, State* st) {
-    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+__device__ inline bool consume_convergence(const double* __restrict__ part, int nblk, int mode, State* st,
+                                           bool first_block) {
+    const bool lead = first_block && threadIdx.x == 0 && threadIdx.y == 0;
     double tot[3] = {0.0, 0.0, 0.0};
     if (mode == CONS_START) {
         block_sum_partials<3>(part, nblk, tot);
@@ -223,6 +239,9 @@ __device__ inline bool consume_convergence(const double* __restrict__ part, int 
         st->done = conv || it >= st->maxit;
     }
     return conv;
+}
+__device__ inline bool consume_convergence(const double* __restrict__ part, int nblk, int mode, State* st) {
+    return consume_convergence(part, nblk, mode, st, blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0);
 }
 
 // ---- column addressing ------------------------------------------------------

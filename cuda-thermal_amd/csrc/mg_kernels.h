@@ -150,32 +150,47 @@ __device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __rest
     cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
 }
 
+// Block (B.x, B.y) covers kColsX x blockDim.y coarse cells of plane B.z.
+template <class T>
+__device__ __forceinline__ void restrict_fast_body(const Blk& B, const Geo& f, const Ops<T>& o,
+                                                   const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb,
+                                                   const State* __restrict__ st) {
+    if (st->done) return;
+    const int I = B.x * kColsX + threadIdx.x;
+    const int J = B.y * blockDim.y + threadIdx.y;
+    if (I >= cg.nx || J >= cg.ny) return;
+    restrict_cell_fast<T>(f, o.gE, o.gN, fx, cg, cb, I, J, B.z);
+}
 template <class T>
 __global__ __launch_bounds__(256) void k_restrict_fast(Geo f, Ops<T> o, const T* __restrict__ fx, Geo cg,
                                                         T* __restrict__ cb, const State* __restrict__ st) {
-    if (st->done) return;
-    const int I = blockIdx.x * kColsX + threadIdx.x;
-    const int J = blockIdx.y * kColsY + threadIdx.y;
-    if (I >= cg.nx || J >= cg.ny) return;
-    restrict_cell_fast<T>(f, o.gE, o.gN, fx, cg, cb, I, J, blockIdx.z);
+    restrict_fast_body<T>(hw_blk(), f, o, fx, cg, cb, st);
 }
 
-// One colour sweep of a level. The first sweep of a V-cycle can also carry
-// the convergence test of the kernel before it (cons_mode, see
+// One colour sweep of a level: block (B.x, B.y) covers kColsX columns of the
+// colour in each of blockDim.y rows. The first sweep of a V-cycle can also
+// carry the convergence test of the kernel before it (cons_mode, see
 // consume_convergence): a converged solve ends here, with no separate launch.
+template <class T, int NZ, int MODE>
+__device__ __forceinline__ void sweep_body(const Blk& B, const Geo& g, const Ops<T>& o, const T* __restrict__ finv,
+                                           const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
+                                           const T* __restrict__ xc, State* st, const double* __restrict__ cons_part,
+                                           int cons_nblk, int cons_mode) {
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
+    const int t = B.x * kColsX + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
+    if (j >= g.ny) return;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= g.nx) return;
+    sweep_column<T, NZ, MODE>(g, o.gE, o.gN, o.gU, finv, b, x, i, j, gc, xc);
+}
 template <class T, int NZ, int MODE>
 __global__ __launch_bounds__(256) void k_sweep(Geo g, Ops<T> o, const T* __restrict__ finv,
                                                 const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc,
                                                 const T* __restrict__ xc, State* st, const double* __restrict__ cons_part,
                                                 int cons_nblk, int cons_mode) {
-    if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
-    const int t = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
-    if (j >= g.ny) return;
-    const int i = 2 * t + ((j + colour) & 1);
-    if (i >= g.nx) return;
-    sweep_column<T, NZ, MODE>(g, o.gE, o.gN, o.gU, finv, b, x, i, j, gc, xc);
+    sweep_body<T, NZ, MODE>(hw_blk(), g, o, finv, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 template <class T>
@@ -304,12 +319,10 @@ inline size_t tail_lds_bytes(const Geo* g, int n) {
 // (~100-cycle) instead of L2 latency. Only the top level's x goes back to
 // global memory, for the prolongation into the level above.
 template <int NZ>
-__global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, State* st, const double* __restrict__ cons_part,
-                                                   int cons_nblk, int cons_mode) {
-    extern __shared__ float sm[];
-    __shared__ TailLevel<float> S[kMaxTail];  // level views into LDS (not in scratch)
+__device__ __forceinline__ void tail_lds_body(const Tail<float>& t, State* st, const double* __restrict__ cons_part,
+                                              int cons_nblk, int cons_mode, float* sm, TailLevel<float>* S) {
     if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
     size_t off = 0;
     for (int k = 0; k < t.n; ++k) {
         const TailLevel<float>& G = t.L[k];
@@ -334,6 +347,13 @@ __global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, Sta
     tail_vcycle<float, NZ, kTailLdsThreads>(S, t.n, 0.0f, 1);
     const long cells0 = t.L[0].g.cells();
     for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = S[0].x[q];
+}
+template <int NZ>
+__global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, State* st, const double* __restrict__ cons_part,
+                                                   int cons_nblk, int cons_mode) {
+    extern __shared__ float sm[];
+    __shared__ TailLevel<float> S[kMaxTail];  // level views into LDS (not in scratch)
+    tail_lds_body<NZ>(t, st, cons_part, cons_nblk, cons_mode, sm, S);
 }
 
 }  // namespace th

@@ -32,14 +32,14 @@ enum {
 // r.r, b.b (and r.z). RES_CHECK: the true residual ||P - A x|| of the steady
 // operator, no writes.
 template <class T, int MODE>
-__global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
-                                                   const double* __restrict__ x, double* __restrict__ r,
-                                                   T* __restrict__ b0, const double* __restrict__ invd,
-                                                   double* __restrict__ z, double* __restrict__ p,
-                                                   const State* __restrict__ st, double* __restrict__ part) {
+__device__ __forceinline__ void residual_body(const Blk& B, const Geo& g, const Ops<T>& o, const double* __restrict__ b,
+                                              const double* __restrict__ x, double* __restrict__ r,
+                                              T* __restrict__ b0, const double* __restrict__ invd,
+                                              double* __restrict__ z, double* __restrict__ p,
+                                              const State* __restrict__ st, double* __restrict__ part) {
     if (MODE != RES_CHECK && st->done) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int q = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     double v[3] = {0.0, 0.0, 0.0};  // rr, bb, rz
     const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
     int i;
@@ -63,17 +63,25 @@ __global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double
             }
         }
     }
-    block_partials<3>(v, part);
+    block_partials<3>(v, part, B.flat());
+}
+template <class T, int MODE>
+__global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double* __restrict__ b,
+                                                   const double* __restrict__ x, double* __restrict__ r,
+                                                   T* __restrict__ b0, const double* __restrict__ invd,
+                                                   double* __restrict__ z, double* __restrict__ p,
+                                                   const State* __restrict__ st, double* __restrict__ part) {
+    residual_body<T, MODE>(hw_blk(), g, o, b, x, r, b0, invd, z, p, st, part);
 }
 
 // Ap = (A + C/dt) p; partial p.Ap.
 template <class T, int NZ>
-__global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
-                                                double* __restrict__ Ap, const State* __restrict__ st,
-                                                double* __restrict__ part) {
+__device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops<T>& o, const double* __restrict__ p,
+                                           double* __restrict__ Ap, const State* __restrict__ st,
+                                           double* __restrict__ part) {
     if (st->done) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int q = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     double v[1] = {0.0};
     const double inv_dt = st->inv_dt;
     const T* __restrict__ gE = o.gE;
@@ -116,7 +124,13 @@ __global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* _
             }
         }
     }
-    block_partials<1>(v, part);
+    block_partials<1>(v, part, B.flat());
+}
+template <class T, int NZ>
+__global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
+                                                double* __restrict__ Ap, const State* __restrict__ st,
+                                                double* __restrict__ part) {
+    apply_body<T, NZ>(hw_blk(), g, o, p, Ap, st, part);
 }
 
 // x += alpha p, r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or
@@ -125,19 +139,19 @@ __global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* _
 // Multigrid path: alpha = rz / p.Ap is formed in every block from the apply
 // kernel's partials (apart), so no reduction launch sits between the two.
 template <class T, int MODE>
-__global__ __launch_bounds__(kVecThreads) void k_update(long n, const double* __restrict__ p,
-                                                         const double* __restrict__ Ap, double* __restrict__ x,
-                                                         double* __restrict__ r, T* __restrict__ b0,
-                                                         const T* __restrict__ zold, const double* __restrict__ invd,
-                                                         double* __restrict__ z, State* st, double* __restrict__ part,
-                                                         const double* __restrict__ apart, int anblk) {
+__device__ __forceinline__ void update_body(const Blk& B, long n, const double* __restrict__ p,
+                                            const double* __restrict__ Ap, double* __restrict__ x,
+                                            double* __restrict__ r, T* __restrict__ b0,
+                                            const T* __restrict__ zold, const double* __restrict__ invd,
+                                            double* __restrict__ z, State* st, double* __restrict__ part,
+                                            const double* __restrict__ apart, int anblk) {
     if (st->done) return;
     double alpha;
     if (apart) {
         double pAp[1];
         block_sum_partials<1>(apart, anblk, pAp);
         alpha = st->rzb[st->iters & 1] / pAp[0];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (B.x == 0 && threadIdx.x == 0) {
             st->pAp = pAp[0];
             st->alpha = alpha;
         }
@@ -145,8 +159,8 @@ __global__ __launch_bounds__(kVecThreads) void k_update(long n, const double* __
         alpha = st->alpha;
     }
     double v[2] = {0.0, 0.0};
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const long stride = (long)B.gx * blockDim.x;
+    for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         x[k] += alpha * p[k];
         const double ri = r[k] - alpha * Ap[k];
         r[k] = ri;
@@ -160,18 +174,32 @@ __global__ __launch_bounds__(kVecThreads) void k_update(long n, const double* __
             b0[k] = (T)ri;
         }
     }
-    block_partials<2>(v, part);
+    block_partials<2>(v, part, B.x);
+}
+template <class T, int MODE>
+__global__ __launch_bounds__(kVecThreads) void k_update(long n, const double* __restrict__ p,
+                                                         const double* __restrict__ Ap, double* __restrict__ x,
+                                                         double* __restrict__ r, T* __restrict__ b0,
+                                                         const T* __restrict__ zold, const double* __restrict__ invd,
+                                                         double* __restrict__ z, State* st, double* __restrict__ part,
+                                                         const double* __restrict__ apart, int anblk) {
+    update_body<T, MODE>(hw_blk(), n, p, Ap, x, r, b0, zold, invd, z, st, part, apart, anblk);
 }
 
 // Partial r.z with z = the V-cycle output.
 template <class T>
-__global__ __launch_bounds__(kVecThreads) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
-                                                      const State* __restrict__ st, double* __restrict__ part) {
+__device__ __forceinline__ void dot_body(const Blk& B, long n, const double* __restrict__ r, const T* __restrict__ z,
+                                         const State* __restrict__ st, double* __restrict__ part) {
     if (st->done) return;
     double v[1] = {0.0};
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) v[0] += r[k] * (double)z[k];
-    block_partials<1>(v, part);
+    const long stride = (long)B.gx * blockDim.x;
+    for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) v[0] += r[k] * (double)z[k];
+    block_partials<1>(v, part, B.x);
+}
+template <class T>
+__global__ __launch_bounds__(kVecThreads) void k_dot(long n, const double* __restrict__ r, const T* __restrict__ z,
+                                                      const State* __restrict__ st, double* __restrict__ part) {
+    dot_body<T>(hw_blk(), n, r, z, st, part);
 }
 
 // p = z + beta p (first: p = z). Multigrid path: every block forms r.z from
@@ -179,8 +207,8 @@ __global__ __launch_bounds__(kVecThreads) void k_dot(long n, const double* __res
 // rz_new / rz_old or flexible (rz_new - r.z_old) / rz_old - with rz kept by
 // iteration parity, so blocks never read a value another block rewrites.
 template <class Z, int FIRST>
-__global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
-                                                          State* st, const double* __restrict__ dpart, int dnblk) {
+__device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __restrict__ z, double* __restrict__ p,
+                                             State* st, const double* __restrict__ dpart, int dnblk) {
     if (st->done) return;
     double beta = 0.0;
     if (dpart) {
@@ -191,7 +219,7 @@ __global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __rest
             const double old = st->rzb[(it + 1) & 1];
             beta = st->flexible ? (rz[0] - st->rzo) / old : rz[0] / old;
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (B.x == 0 && threadIdx.x == 0) {
             st->rzb[it & 1] = rz[0];
             st->rz = rz[0];
             st->beta = beta;
@@ -199,9 +227,14 @@ __global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __rest
     } else {
         beta = st->beta;
     }
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    const long stride = (long)B.gx * blockDim.x;
+    for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride)
         p[k] = FIRST ? (double)z[k] : (double)z[k] + beta * p[k];
+}
+template <class Z, int FIRST>
+__global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
+                                                          State* st, const double* __restrict__ dpart, int dnblk) {
+    pupdate_body<Z, FIRST>(hw_blk(), n, z, p, st, dpart, dnblk);
 }
 
 // Adds the partials of the previous kernel and runs its epilogue.
@@ -265,7 +298,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
 }
 
 // Start of a solve: tolerance, iteration cap, cleared flags.
-__global__ void k_begin(State* st, double tol, int maxit) {
+static __global__ void k_begin(State* st, double tol, int maxit) {
     st->tol = tol;
     st->maxit = maxit;
     st->iters = 0;
@@ -275,20 +308,20 @@ __global__ void k_begin(State* st, double tol, int maxit) {
     st->relres = 0.0;
 }
 // Reopens a finished solve for a restart from the current x (iterations kept).
-__global__ void k_reopen(State* st) {
+static __global__ void k_reopen(State* st) {
     st->done = 0;
     st->converged = 0;
 }
-__global__ void k_set_inv_dt(State* st, double inv_dt) {
+static __global__ void k_set_inv_dt(State* st, double inv_dt) {
     st->inv_dt = inv_dt;
     st->refactor = 1;
 }
-__global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
+static __global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
 
 // Transient stepping on the device (transient3d.h:57-84), so that a step is
 // a replayable graph whose size of step is not a launch argument.
 // k_step_init: t = 0, first dt, stepping rule, heat content of x = 0.
-__global__ void k_step_init(State* st, double dt0, double growth, double dt_max, double t_end) {
+static __global__ void k_step_init(State* st, double dt0, double growth, double dt_max, double t_end) {
     st->t = 0.0;
     st->dt = -1.0;
     st->dt_next = dt0;
@@ -301,7 +334,7 @@ __global__ void k_step_init(State* st, double dt0, double growth, double dt_max,
 // dt = min(dt, t_end - t) (transient3d.h:60); a changed dt sets the new
 // operator's 1/dt and asks for new factors (set_dt only if dt changed, :61).
 // Then tolerance and flags of the step's solve, as k_begin.
-__global__ void k_step_begin(State* st, double tol, int maxit) {
+static __global__ void k_step_begin(State* st, double tol, int maxit) {
     const double dt = fmin(st->dt_next, st->t_end - st->t);
     st->refactor = dt != st->dt;
     if (st->refactor) {
@@ -319,7 +352,7 @@ __global__ void k_step_begin(State* st, double tol, int maxit) {
 // After the step's energy terms (k_finalize FIN_ENERGY): t += dt, the step's
 // record with q_stored = (sum C T_new - sum C T_old) / dt, and the next dt
 // = min(dt * growth, dt_max) (transient3d.h:73-84).
-__global__ void k_step_end(State* st) {
+static __global__ void k_step_end(State* st) {
     const double dt = st->dt;
     st->t += dt;
     st->last.t = st->t;
@@ -334,7 +367,7 @@ __global__ void k_step_end(State* st) {
 }
 
 // x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).
-__global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
+static __global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
     if (!st->zero_x) return;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = 0.0;
@@ -408,7 +441,7 @@ struct LiftMap {
 // x(i, j, l) = scale * the coarse field at the centre of cell (i, j),
 // bilinear over the Hc x Wc coarse cells (clamped at the edges), then
 // linear in z through the map.
-__global__ void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C, LiftMap zm, double scale,
+static __global__ void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C, LiftMap zm, double scale,
                        double* __restrict__ x) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)g.nx * g.ny * g.nz) return;
@@ -489,7 +522,7 @@ __global__ void k_to_split(Geo g, int planes, const S* __restrict__ in, D* __res
     const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
     out[(long)l * g.plane + g.pos(i, j)] = (D)in[k];
 }
-__global__ void k_to_natural(Geo g, const double* __restrict__ in, double* __restrict__ out) {
+static __global__ void k_to_natural(Geo g, const double* __restrict__ in, double* __restrict__ out) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)g.nx * g.ny * g.nz;
     if (k >= n) return;
@@ -503,7 +536,7 @@ __global__ void k_cast(long n, const S* __restrict__ in, D* __restrict__ out) {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = (D)in[k];
 }
-__global__ void k_mul(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+static __global__ void k_mul(long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) out[k] = a[k] * b[k];
 }

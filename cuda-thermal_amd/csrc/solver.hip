@@ -1,838 +1,6 @@
-// solver.hip - MG-preconditioned CG for layered stacks on MI355X (gfx950),
-// and the C ABI of the solver (include/thermal_hip.h).
-//
-// Follows the reference's GPU/CPU solver (src/pcg3d_gpu.cu, src/pcg3d_cpu.cpp,
-// src/mg3d.h) in what it computes; the device design is this file's own:
-//  * every vector and level in one colour-split layout (kernels.h), so PCG
-//    and multigrid hand vectors over without re-layout kernels;
-//  * one thread per column for stencil work, vertical neighbours in registers;
-//  * the V-cycle needs no clearing pass: the first (red) pre-smoothing sweep
-//    runs with the black neighbours taken as zero, which is what the
-//    reference's memset-then-sweep computes, and prolongation is folded into
-//    the first post-smoothing sweep, which adds the parent's correction to
-//    each neighbour it reads (the cells it writes are overwritten anyway);
-//  * only 1/pivot is stored per cell: the upper Thomas factor is rebuilt in
-//    the sweep as (-gU) * (1/pivot), the same product the reference stores;
-//  * PCG scalars, tolerance test and iteration count live on the device;
-//    every kernel returns at once when the solve is done, so iterations can
-//    be queued ahead of the host's convergence check and replayed as graphs.
-// Precision is a runtime choice: T = float (FP32 conductances, factors and
-// V-cycle inside the FP64 PCG) or T = double (FP64 throughout).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "kernels.h"
-#include "levels.h"
-#include "mg_kernels.h"
-#include "model.h"
-#include "pcg_kernels.h"
-
-namespace th {
-
-// ===========================================================================
-// Host side
-// ===========================================================================
-
-#define HIPTRY(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess) return fail(TH_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-static dim3 col_block() { return dim3(kColsX, kColsY); }
-static dim3 col_grid(int slots, int rows) {
-    return dim3((unsigned)((slots + kColsX - 1) / kColsX), (unsigned)((rows + kColsY - 1) / kColsY));
-}
-static unsigned vec_blocks(long n) {
-    return (unsigned)std::max(1L, std::min((long)kMaxVecBlocks, (n + kVecThreads - 1) / kVecThreads));
-}
-static unsigned lin_blocks(long n) { return (unsigned)((n + 255) / 256); }
-static dim3 stencil_block() { return dim3(kColsX, kStencilRows); }
-static dim3 stencil_grid(int slots, int rows) {
-    return dim3((unsigned)((slots + kColsX - 1) / kColsX), (unsigned)((rows + kStencilRows - 1) / kStencilRows));
-}
-
-template <class T>
-struct DevLevel {
-    Geo g;
-    T *gE = nullptr, *gN = nullptr, *gU = nullptr, *gB = nullptr, *cap = nullptr;
-    T *finv = nullptr, *x = nullptr, *b = nullptr;
-    Ops<T> ops() const { return {gE, gN, gU, gB, cap}; }
-};
-
-struct SolverBase {
-    virtual ~SolverBase() {}
-    virtual int init(const th_model_desc& m, const th_solver_opts& o) = 0;
-    virtual int set_dt(double dt) = 0;
-    virtual int set_x_zero() = 0;
-    virtual int set_x(const double* x) = 0;
-    virtual int set_rhs_steady() = 0;
-    virtual int set_rhs_transient() = 0;
-    virtual int solve(double tol, int maxit, th_stats* st) = 0;
-    virtual int get_x(double* x) = 0;
-    virtual int set_power(const double* P) = 0;
-    virtual int device_x(const double** p) = 0;
-    virtual int true_relres(double* out) = 0;
-    virtual int energy(double* heat, double* out, double* tmax) = 0;
-    virtual int nlevels() const = 0;
-    virtual hipStream_t stream() const = 0;
-    virtual int precondition(const double* r, double* z) = 0;
-    virtual int apply_operator(const double* x, double* y) = 0;
-    virtual int time_kernel(int which, int reps, double* avg_ms, double* bytes) = 0;
-    virtual int run_transient(const th_transient_opts& o, th_step_record* recs, int rec_cap, int* nsteps,
-                              int* all_converged, double* ms) = 0;
-    virtual int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
-                             int* used, double* rp, double* rl) = 0;
-    virtual int nz() const = 0;
-};
-
-template <class T>
-struct Solver final : SolverBase {
-    th_solver_opts opt{};
-    hipStream_t s = nullptr;
-    Geo g0{};
-    long n0 = 0;       // cells in the split layout (with padding)
-    long nnat = 0;     // cells in the natural layout
-    double inv_dt = 0;
-    double *P = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
-    double *z = nullptr, *invd = nullptr, *xnat = nullptr;
-    double* part = nullptr;
-    State* st = nullptr;
-    State* h_st = nullptr;  // pinned mirror
-    std::vector<DevLevel<T>> L;
-    hipGraphExec_t g_start = nullptr, g_iter = nullptr;
-    std::vector<void*> owned;
-
-    ~Solver() override {
-        if (s) (void)hipStreamSynchronize(s);
-        if (g_start) (void)hipGraphExecDestroy(g_start);
-        if (g_iter) (void)hipGraphExecDestroy(g_iter);
-        for (void* ptr : owned) (void)hipFree(ptr);
-        if (h_st) (void)hipHostFree(h_st);
-        if (s) (void)hipStreamDestroy(s);
-    }
-
-    template <class U>
-    int dalloc(U** ptr, size_t count) {
-        void* v = nullptr;
-        HIPTRY(hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(U)));
-        owned.push_back(v);
-        HIPTRY(hipMemsetAsync(v, 0, std::max<size_t>(count, 1) * sizeof(U), s));
-        *ptr = (U*)v;
-        return TH_OK;
-    }
-
-    bool mg() const { return opt.precond == TH_PRECOND_MG; }
-
-    // natural-layout host array -> split-layout device array of type D
-    template <class S, class D>
-    int upload_split(const Geo& g, int planes, const S* host, D* dev) {
-        const long n = (long)g.nx * g.ny * planes;
-        void* tmp = nullptr;
-        HIPTRY(hipMalloc(&tmp, n * sizeof(S)));
-        hipError_t e = hipMemcpyAsync(tmp, host, n * sizeof(S), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) {
-            k_to_split<S, D><<<lin_blocks(n), 256, 0, s>>>(g, planes, (const S*)tmp, dev);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
-        return TH_OK;
-    }
-
-    int init(const th_model_desc& m, const th_solver_opts& o) override {
-        opt = o;
-        HIPTRY(hipSetDevice(o.device));
-        HIPTRY(hipStreamCreate(&s));
-        g0 = make_geo(m.nx, m.ny, m.nz);
-        n0 = g0.cells();
-        nnat = (long)m.nx * m.ny * m.nz;
-        int rc = 0;
-        for (double** v : {&P, &x, &b, &r, &p, &Ap})
-            if ((rc = dalloc(v, n0))) return rc;
-        if ((rc = dalloc(&xnat, nnat))) return rc;
-        if (!mg())
-            for (double** v : {&z, &invd})
-                if ((rc = dalloc(v, n0))) return rc;
-        if ((rc = dalloc(&st, 1))) return rc;
-        HIPTRY(hipHostMalloc((void**)&h_st, sizeof(State), hipHostMallocDefault));
-        std::memset(h_st, 0, sizeof(State));
-
-        // Levels: coarse operators aggregated on the host (levels.cpp), each
-        // level stored in the split layout in precision T.
-        std::vector<HostLevel> H;
-        if (mg()) {
-            H = build_hierarchy(m.nx, m.ny, m.nz, m.dx, m.dy, m.gE, m.gN, m.gU, m.gB, m.cap);
-        } else {  // Jacobi still needs the fine operator
-            HostLevel f;
-            f.nx = m.nx;
-            f.ny = m.ny;
-            f.nz = m.nz;
-            H.push_back(f);
-        }
-        L.resize(H.size());
-        long maxblk = kMaxVecBlocks;
-        for (size_t k = 0; k < H.size(); ++k) {
-            DevLevel<T>& d = L[k];
-            d.g = make_geo(H[k].nx, H[k].ny, H[k].nz);
-            const long nc = d.g.cells();
-            for (T** v : {&d.gE, &d.gN, &d.gU, &d.cap, &d.finv, &d.x, &d.b})
-                if ((rc = dalloc(v, nc))) return rc;
-            if ((rc = dalloc(&d.gB, d.g.plane))) return rc;
-            const float* src[5];
-            if (k == 0) {
-                src[0] = m.gE, src[1] = m.gN, src[2] = m.gU, src[3] = m.cap, src[4] = m.gB;
-            } else {
-                src[0] = H[k].gE.data(), src[1] = H[k].gN.data(), src[2] = H[k].gU.data(),
-                src[3] = H[k].cap.data(), src[4] = H[k].gB.data();
-            }
-            T* dst[5] = {d.gE, d.gN, d.gU, d.cap, d.gB};
-            for (int a = 0; a < 5; ++a)
-                if ((rc = upload_split<float, T>(d.g, a == 4 ? 1 : d.g.nz, src[a], dst[a]))) return rc;
-            const dim3 cg = col_grid(2 * d.g.hx, d.g.ny);
-            maxblk = std::max(maxblk, (long)cg.x * cg.y);
-            if (!mg()) break;
-        }
-        if ((rc = dalloc(&part, maxblk * 4))) return rc;
-        {
-            const dim3 sg = stencil_grid(2 * g0.hx, g0.ny);
-            nblk_stencil = sg.x * sg.y;
-            nblk_vec = vec_blocks(n0);
-            if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_stencil)) ||
-                (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
-                return rc;
-        }
-        // The lowest levels run in one workgroup. FP32 with the lateral
-        // restriction and nz <= 16: as many levels as fit in LDS (k_tail_lds).
-        // Otherwise: levels with at most kTailCols columns (k_tail, L2).
-        fast = opt.restriction == TH_RESTRICT_LATERAL;
-        tail_k0 = L.size() - 1;
-        lds_tail = fast && sizeof(T) == 4 && g0.nz <= 16;
-        if (lds_tail) {
-            auto bytes_from = [&](size_t k0) {
-                std::vector<Geo> gs;
-                for (size_t k = k0; k < L.size(); ++k) gs.push_back(L[k].g);
-                return tail_lds_bytes(gs.data(), (int)gs.size());
-            };
-            while (tail_k0 > 0 && bytes_from(tail_k0 - 1) <= kTailLdsBytes && L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
-                --tail_k0;
-            lds_bytes = bytes_from(tail_k0);
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<8>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<12>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<16>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-        } else {
-            while (tail_k0 > 0 && (long)L[tail_k0 - 1].g.nx * L[tail_k0 - 1].g.ny <= kTailCols &&
-                   L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
-                --tail_k0;
-        }
-        if ((rc = upload_split<double, double>(g0, g0.nz, m.P, P))) return rc;
-        if ((rc = set_dt(0.0))) return rc;
-        HIPTRY(hipStreamSynchronize(s));
-        if (opt.use_graphs && mg()) {
-            if ((rc = capture(&g_start, true))) return rc;
-            if ((rc = capture(&g_iter, false))) return rc;
-        }
-        return TH_OK;
-    }
-
-    int capture(hipGraphExec_t* exec, bool start) {
-        return capture_fn(exec, [&]() { return start ? enqueue_start() : enqueue_iter(); });
-    }
-    template <class F>
-    int capture_fn(hipGraphExec_t* exec, F&& enqueue) {
-        hipGraph_t gr = nullptr;
-        HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue();
-        hipError_t e = hipStreamEndCapture(s, &gr);
-        if (rc) {
-            if (gr) (void)hipGraphDestroy(gr);
-            return rc;
-        }
-        if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-        return TH_OK;
-    }
-
-    int nlevels() const override { return mg() ? (int)L.size() : 0; }
-    hipStream_t stream() const override { return s; }
-
-    int set_dt(double dt) override {
-        inv_dt = dt > 0 ? 1.0 / dt : 0.0;
-        k_set_inv_dt<<<1, 1, 0, s>>>(st, inv_dt);
-        HIPTRY(hipGetLastError());
-        return enqueue_factors();
-    }
-    // Level factors (MG) or 1/diag (Jacobi) for the state's 1/dt; the kernels
-    // return at once unless the state asks for new factors.
-    int enqueue_factors() {
-        if (mg()) {
-            for (DevLevel<T>& d : L) {
-                k_factor<T><<<col_grid(2 * d.g.hx, d.g.ny), col_block(), 0, s>>>(d.g, d.ops(), d.finv, st);
-                HIPTRY(hipGetLastError());
-            }
-        } else {
-            const DevLevel<T>& d = L[0];
-            k_invdiag<T><<<col_grid(2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, d.ops(), invd, st);
-            HIPTRY(hipGetLastError());
-        }
-        return TH_OK;
-    }
-    int set_x_zero() override {
-        HIPTRY(hipMemsetAsync(x, 0, n0 * sizeof(double), s));
-        return TH_OK;
-    }
-    int set_x(const double* host) override { return upload_split<double, double>(g0, g0.nz, host, x); }
-    int set_rhs_steady() override {
-        HIPTRY(hipMemcpyAsync(b, P, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        return TH_OK;
-    }
-    int set_rhs_transient() override {
-        k_rhs_transient<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, L[0].cap, P, x, b, st);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-    int set_power(const double* host) override {
-        HIPTRY(hipStreamSynchronize(s));
-        return upload_split<double, double>(g0, g0.nz, host, P);
-    }
-    int device_x(const double** out) override {
-        k_to_natural<<<lin_blocks(nnat), 256, 0, s>>>(g0, x, xnat);
-        HIPTRY(hipGetLastError());
-        HIPTRY(hipStreamSynchronize(s));
-        *out = xnat;
-        return TH_OK;
-    }
-    int get_x(double* host) override {
-        k_to_natural<<<lin_blocks(nnat), 256, 0, s>>>(g0, x, xnat);
-        HIPTRY(hipGetLastError());
-        HIPTRY(hipMemcpyAsync(host, xnat, nnat * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPTRY(hipStreamSynchronize(s));
-        return TH_OK;
-    }
-
-    // ---- launch sequences -------------------------------------------------
-
-    size_t tail_k0 = 0;     // first level handled by the single-workgroup tail
-    bool fast = true;       // lateral (fast) restriction
-    bool lds_tail = false;  // tail levels held in LDS
-    size_t lds_bytes = 0;
-
-    // Partial buffers, one per producing kernel, so a consumer can still read
-    // its producer's partials while the next reduction runs.
-    double *part_res = nullptr, *part_apply = nullptr, *part_upd = nullptr, *part_dot = nullptr;
-    unsigned nblk_stencil = 0, nblk_vec = 0;
-
-    template <int NV, int EPI, unsigned KMAX = 0>
-    int finalize(unsigned nblk, const double* pp = nullptr) {
-        k_finalize<NV, EPI, KMAX><<<1, kFinThreads, 0, s>>>(pp ? pp : part, (int)nblk, st);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-
-    // Consumer of the previous kernel's partials, carried by the first kernel
-    // of the V-cycle (see consume_convergence).
-    struct Cons {
-        const double* part;
-        int nblk;
-        int mode;
-    };
-
-    template <int MODE>
-    int launch_sweep(size_t k, int colour, Cons cs = {nullptr, 0, CONS_NONE}) {
-        const DevLevel<T>& d = L[k];
-        const DevLevel<T>& c = L[std::min(k + 1, L.size() - 1)];
-        const dim3 grid = col_grid(d.g.hx, d.g.ny);
-#define TH_SWEEP(NZ)                                                                                         \
-    k_sweep<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, d.b, d.x, colour, c.g, c.x, st, \
-                                                      cs.part, cs.nblk, cs.mode)
-        if (d.g.nz <= 4) TH_SWEEP(4);
-        else if (d.g.nz <= 8) TH_SWEEP(8);
-        else if (d.g.nz <= 12) TH_SWEEP(12);
-        else if (d.g.nz <= 16) TH_SWEEP(16);
-        else if (d.g.nz <= 24) TH_SWEEP(24);
-        else if (d.g.nz <= 32) TH_SWEEP(32);
-        else if (d.g.nz <= 48) TH_SWEEP(48);
-        else TH_SWEEP(64);
-#undef TH_SWEEP
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-
-    int launch_tail(Cons cs) {
-        Tail<T> t{};
-        t.n = (int)(L.size() - tail_k0);
-        for (int k = 0; k < t.n; ++k) {
-            const DevLevel<T>& d = L[tail_k0 + k];
-            t.L[k] = TailLevel<T>{d.g, d.ops(), d.finv, d.x, d.b};
-        }
-        const int nz = g0.nz;
-        if (lds_tail) {
-            const Tail<float>& tf = *reinterpret_cast<const Tail<float>*>(&t);
-#define TH_TAIL(NZ) k_tail_lds<NZ><<<1, kTailLdsThreads, lds_bytes, s>>>(tf, st, cs.part, cs.nblk, cs.mode)
-            if (nz <= 4) TH_TAIL(4);
-            else if (nz <= 8) TH_TAIL(8);
-            else if (nz <= 12) TH_TAIL(12);
-            else TH_TAIL(16);
-#undef TH_TAIL
-            HIPTRY(hipGetLastError());
-            return TH_OK;
-        }
-#define TH_TAIL(NZ) \
-    k_tail<T, NZ><<<1, tail_threads<T, NZ>(), 0, s>>>(t, st, cs.part, cs.nblk, cs.mode, fast ? 1 : 0)
-        if (nz <= 4) TH_TAIL(4);
-        else if (nz <= 8) TH_TAIL(8);
-        else if (nz <= 12) TH_TAIL(12);
-        else if (nz <= 16) TH_TAIL(16);
-        else if (nz <= 24) TH_TAIL(24);
-        else if (nz <= 32) TH_TAIL(32);
-        else if (nz <= 48) TH_TAIL(48);
-        else TH_TAIL(64);
-#undef TH_TAIL
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-
-    int launch_restrict(size_t k) {
-        const DevLevel<T>& f = L[k];
-        const DevLevel<T>& c = L[k + 1];
-        dim3 grid = col_grid(c.g.nx, c.g.ny);
-        grid.z = c.g.nz;
-        if (fast)
-            k_restrict_fast<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.x, c.g, c.b, st);
-        else
-            k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-
-    // One V-cycle on level k (pcg3d_cpu.cpp:69-113): pre-smooth red, black;
-    // restrict; coarse V-cycle; post-smooth black (with the prolongation),
-    // red. Level k's x starts from zero. Levels from tail_k0 down run in the
-    // single-workgroup tail kernel. `cs`: convergence test carried by the
-    // V-cycle's first kernel (level 0 only).
-    int vcycle(size_t k, Cons cs = {nullptr, 0, CONS_NONE}) {
-        int rc;
-        if (k == tail_k0) return launch_tail(cs);
-        if ((rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
-        if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
-        if ((rc = launch_restrict(k))) return rc;
-        if ((rc = vcycle(k + 1))) return rc;
-        if ((rc = launch_sweep<SW_PROLONG>(k, 1))) return rc;
-        return launch_sweep<SW_PLAIN>(k, 0);
-    }
-
-    // Ap = (A + C/dt) p with partials p.Ap in part_apply.
-    int launch_apply_kernel() {
-        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        const Ops<T> o = L[0].ops();
-#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, stencil_block(), 0, s>>>(g0, o, p, Ap, st, part_apply)
-        if (g0.nz <= 4) TH_APPLY(4);
-        else if (g0.nz <= 8) TH_APPLY(8);
-        else if (g0.nz <= 12) TH_APPLY(12);
-        else if (g0.nz <= 16) TH_APPLY(16);
-        else if (g0.nz <= 24) TH_APPLY(24);
-        else if (g0.nz <= 32) TH_APPLY(32);
-        else if (g0.nz <= 48) TH_APPLY(48);
-        else TH_APPLY(64);
-#undef TH_APPLY
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-    int launch_apply() {  // with alpha in the state (Jacobi path and tests)
-        int rc;
-        if ((rc = launch_apply_kernel())) return rc;
-        return finalize<1, FIN_ALPHA>(nblk_stencil, part_apply);
-    }
-
-    // Start: residual, first preconditioner application, rz, p = z.
-    int enqueue_start() {
-        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        const unsigned vb = vec_blocks(n0);
-        int rc;
-        if (mg()) {
-            k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
-                                                                   nullptr, st, part_res);
-            HIPTRY(hipGetLastError());
-            if ((rc = vcycle(0, Cons{part_res, (int)nblk_stencil, CONS_START}))) return rc;
-            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
-            HIPTRY(hipGetLastError());
-            k_pupdate<T, 1><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
-            HIPTRY(hipGetLastError());
-            return TH_OK;
-        }
-        k_residual<T, RES_JACOBI><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
-                                                                   part_res);
-        HIPTRY(hipGetLastError());
-        return finalize<3, FIN_RES_JACOBI>(nblk_stencil, part_res);
-    }
-
-    // One PCG iteration: Ap; x, r update (alpha formed from the apply
-    // partials); convergence test + z = M^-1 r (V-cycle); r.z; p = z + beta p
-    // (beta formed from the dot partials). Multigrid: 3 + V-cycle launches.
-    int enqueue_iter() {
-        const unsigned vb = vec_blocks(n0);
-        int rc;
-        if (mg()) {
-            if ((rc = launch_apply_kernel())) return rc;
-            if (opt.beta == TH_BETA_FLEXIBLE)
-                k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, L[0].x, nullptr, nullptr,
-                                                                   st, part_upd, part_apply, (int)nblk_stencil);
-            else
-                k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, nullptr, nullptr, nullptr, st,
-                                                              part_upd, part_apply, (int)nblk_stencil);
-            HIPTRY(hipGetLastError());
-            if ((rc = vcycle(0, Cons{part_upd, (int)vb, CONS_ITER}))) return rc;
-            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
-            HIPTRY(hipGetLastError());
-            k_pupdate<T, 0><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
-            HIPTRY(hipGetLastError());
-            return TH_OK;
-        }
-        if ((rc = launch_apply())) return rc;
-        k_update<T, UP_JACOBI><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, nullptr, nullptr, invd, z, st, part_upd,
-                                                          nullptr, 0);
-        HIPTRY(hipGetLastError());
-        if ((rc = finalize<2, FIN_UPDATE_JACOBI>(vb, part_upd))) return rc;
-        k_pupdate<double, 0><<<vb, kVecThreads, 0, s>>>(n0, z, p, st, nullptr, 0);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-
-    int fetch_state() {
-        HIPTRY(hipMemcpyAsync(h_st, st, sizeof(State), hipMemcpyDeviceToHost, s));
-        HIPTRY(hipStreamSynchronize(s));
-        return TH_OK;
-    }
-
-    int run_start() {
-        if (g_start) {
-            HIPTRY(hipGraphLaunch(g_start, s));
-            return TH_OK;
-        }
-        return enqueue_start();
-    }
-    int run_iter() {
-        if (g_iter) {
-            HIPTRY(hipGraphLaunch(g_iter, s));
-            return TH_OK;
-        }
-        return enqueue_iter();
-    }
-
-    // PCG iterations until the device reports done. Iterations are queued in
-    // batches sized from the observed convergence rate, so the host waits
-    // once per batch rather than once per iteration; iterations queued past
-    // convergence return at once. `first`: size of the first batch.
-    int iterate(int maxit, int first = 1) {
-        int rc;
-        double rate = 0.0;  // relres reduction per iteration, once known
-        while (!h_st->done && h_st->iters < maxit) {
-            int k = first;
-            first = 1;
-            if (rate > 0.0 && rate < 1.0 && h_st->relres > 0.0) {
-                const double need = std::log(h_st->tol / h_st->relres) / std::log(rate);
-                k = (int)std::max(1.0, std::min(16.0, std::ceil(need) - 1.0));
-            }
-            k = std::min(k, maxit - h_st->iters);
-            const int it0 = h_st->iters;
-            const double rel0 = h_st->relres;
-            for (int q = 0; q < k; ++q)
-                if ((rc = run_iter())) return rc;
-            if ((rc = fetch_state())) return rc;
-            const int did = h_st->iters - it0;
-            if (did > 0 && rel0 > 0.0 && h_st->relres > 0.0) rate = std::pow(h_st->relres / rel0, 1.0 / did);
-            if (did == 0) break;  // nothing ran: the device is done
-        }
-        return TH_OK;
-    }
-
-    int solve(double tol, int maxit, th_stats* out) override {
-        int rc;
-        k_begin<<<1, 1, 0, s>>>(st, tol, maxit);
-        HIPTRY(hipGetLastError());
-        if ((rc = run_start()) || (rc = fetch_state())) return rc;
-        if ((rc = converge(maxit, 1))) return rc;
-        k_finish<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, st);
-        HIPTRY(hipGetLastError());
-        if (out) {
-            out->iters = h_st->iters;
-            out->relres = h_st->relres;
-            out->converged = h_st->converged;
-        }
-        return TH_OK;
-    }
-
-    // Iterations after the start of a solve (h_st fresh), then the optional
-    // true-residual check: the recurrence ||r|| drifts from ||b - A x|| by
-    // rounding; restart from x with the true residual until that one is
-    // below tol too.
-    int converge(int maxit, int first) {
-        int rc;
-        if (maxit > 0 && (rc = iterate(maxit, first))) return rc;
-        for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
-            k_reopen<<<1, 1, 0, s>>>(st);
-            HIPTRY(hipGetLastError());
-            if ((rc = run_start()) || (rc = fetch_state())) return rc;
-            if (h_st->done) break;  // true residual below tol
-            if (h_st->iters >= maxit) break;
-            if ((rc = iterate(maxit))) return rc;
-        }
-        return TH_OK;
-    }
-
-    // ---- transient (transient3d.h:48-90) -----------------------------------
-    // A step is three launch sequences, each replayed as one graph:
-    //   head: advance dt on the device, new factors if dt changed, b = P +
-    //         C/dt x, start of the solve warm-started from x;
-    //   iterations, queued ahead as in a steady solve;
-    //   tail: energy terms of the new x and the step's record.
-    // The host keeps its own copy of t and dt (the same double arithmetic
-    // as the device) to know when to stop, and waits once per step.
-    int enqueue_step_head(double tol, int maxit) {
-        k_step_begin<<<1, 1, 0, s>>>(st, tol, maxit);
-        HIPTRY(hipGetLastError());
-        int rc;
-        if ((rc = enqueue_factors())) return rc;
-        if ((rc = set_rhs_transient())) return rc;
-        return enqueue_start();
-    }
-    int enqueue_step_tail() {
-        const unsigned vb = vec_blocks(n0);
-        k_finish<<<vb, kVecThreads, 0, s>>>(n0, x, st);
-        HIPTRY(hipGetLastError());
-        k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
-        HIPTRY(hipGetLastError());
-        int rc;
-        if ((rc = finalize<3, FIN_ENERGY, 4u>(vb))) return rc;
-        k_step_end<<<1, 1, 0, s>>>(st);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
-    }
-    int run_transient(const th_transient_opts& o, th_step_record* recs, int rec_cap, int* nsteps, int* all_converged,
-                      double* ms) override {
-        static_assert(sizeof(StepRec) == sizeof(th_step_record), "step record layout");
-        using clk = std::chrono::high_resolution_clock;
-        int rc;
-        hipGraphExec_t gh = nullptr, gt = nullptr;
-        auto cleanup = [&]() {
-            if (gh) (void)hipGraphExecDestroy(gh);
-            if (gt) (void)hipGraphExecDestroy(gt);
-        };
-        if (opt.use_graphs && mg()) {
-            if ((rc = capture_fn(&gh, [&]() { return enqueue_step_head(o.tol, o.maxit); })) ||
-                (rc = capture_fn(&gt, [&]() { return enqueue_step_tail(); }))) {
-                cleanup();
-                return rc;
-            }
-        }
-        auto body = [&]() -> int {
-            if ((rc = set_x_zero())) return rc;
-            k_step_init<<<1, 1, 0, s>>>(st, o.dt0, o.growth, o.dt_max, o.t_end);
-            HIPTRY(hipGetLastError());
-            HIPTRY(hipStreamSynchronize(s));
-            const auto t0 = clk::now();
-            double t = 0.0, dt = o.dt0;
-            int steps = 0, conv = 1, prev_iters = 2;
-            auto take_record = [&](int k) -> int {  // h_st->last is the record of step k
-                if (!recs || k < 0) return TH_OK;
-                if (k >= rec_cap) return fail(TH_ERR_ARG, "record buffer too small");
-                std::memcpy(&recs[k], &h_st->last, sizeof(StepRec));
-                return TH_OK;
-            };
-            while (t < o.t_end * (1 - 1e-12) && (o.max_steps <= 0 || steps < o.max_steps)) {
-                dt = std::min(dt, o.t_end - t);
-                if (gh) HIPTRY(hipGraphLaunch(gh, s));
-                else if ((rc = enqueue_step_head(o.tol, o.maxit))) return rc;
-                for (int q = 0; q < std::min(prev_iters, o.maxit); ++q)
-                    if ((rc = run_iter())) return rc;
-                if ((rc = fetch_state())) return rc;
-                if ((rc = take_record(steps - 1))) return rc;
-                if ((rc = converge(o.maxit, 1))) return rc;
-                if (!h_st->converged) conv = 0;
-                prev_iters = std::max(1, h_st->iters);
-                if (gt) HIPTRY(hipGraphLaunch(gt, s));
-                else if ((rc = enqueue_step_tail())) return rc;
-                t += dt;
-                dt = std::min(dt * o.growth, o.dt_max);
-                ++steps;
-            }
-            if ((rc = fetch_state())) return rc;
-            if ((rc = take_record(steps - 1))) return rc;
-            if (ms) *ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
-            *nsteps = steps;
-            if (all_converged) *all_converged = conv;
-            return TH_OK;
-        };
-        rc = body();
-        cleanup();
-        return rc;
-    }
-
-    // Initial guess from a coarse field with the plain warm start as the
-    // fallback (th_solver_set_x_lifted). Ap holds the plain x meanwhile.
-    int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
-                     int* used, double* rp, double* rl) override {
-        int rc;
-        double r_plain = 0.0, r_lift = 0.0;
-        if ((rc = true_relres(&r_plain))) return rc;
-        HIPTRY(hipMemcpyAsync(Ap, x, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, x);
-        HIPTRY(hipGetLastError());
-        const Ops<T> o = L[0].ops();
-        for (int k = 0; k < passes; ++k)
-            for (int colour = 0; colour < 2; ++colour) {
-                const dim3 grid = col_grid(g0.hx, g0.ny);
-#define TH_RELAX(NZ) k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, x, colour)
-                if (g0.nz <= 4) TH_RELAX(4);
-                else if (g0.nz <= 8) TH_RELAX(8);
-                else if (g0.nz <= 12) TH_RELAX(12);
-                else if (g0.nz <= 16) TH_RELAX(16);
-                else if (g0.nz <= 24) TH_RELAX(24);
-                else if (g0.nz <= 32) TH_RELAX(32);
-                else if (g0.nz <= 48) TH_RELAX(48);
-                else TH_RELAX(64);
-#undef TH_RELAX
-                HIPTRY(hipGetLastError());
-            }
-        if ((rc = true_relres(&r_lift))) return rc;
-        const bool keep = r_lift < r_plain;
-        if (!keep) HIPTRY(hipMemcpyAsync(x, Ap, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPTRY(hipStreamSynchronize(s));
-        if (used) *used = keep;
-        if (rp) *rp = r_plain;
-        if (rl) *rl = r_lift;
-        return TH_OK;
-    }
-    int nz() const override { return g0.nz; }
-
-    int true_relres(double* out) override {
-        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, x, nullptr, nullptr, nullptr,
-                                                                  nullptr, nullptr, st, part);
-        HIPTRY(hipGetLastError());
-        int rc;
-        if ((rc = finalize<3, FIN_CHECK>(grid.x * grid.y)) || (rc = fetch_state())) return rc;
-        *out = std::sqrt(h_st->chk_rr / h_st->chk_bb);
-        return TH_OK;
-    }
-
-    int energy(double* heat, double* outq, double* tmax) override {
-        const unsigned vb = vec_blocks(n0);
-        k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
-        HIPTRY(hipGetLastError());
-        int rc;
-        if ((rc = finalize<3, FIN_ENERGY, 4u>(vb)) || (rc = fetch_state())) return rc;
-        if (heat) *heat = h_st->e_heat;
-        if (outq) *outq = h_st->e_out;
-        if (tmax) *tmax = h_st->e_tmax;
-        return TH_OK;
-    }
-
-    // Test entry points: one preconditioner application / one operator apply
-    // on host vectors. They use r, p and Ap as staging and leave x, b alone.
-    int precondition(const double* host_r, double* host_z) override {
-        int rc;
-        if ((rc = upload_split<double, double>(g0, g0.nz, host_r, r))) return rc;
-        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1);
-        HIPTRY(hipGetLastError());
-        const unsigned vb = vec_blocks(n0);
-        double* zd = Ap;  // z as FP64 in the split layout
-        if (mg()) {
-            k_cast<T, double><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].b);
-            HIPTRY(hipGetLastError());
-            if ((rc = vcycle(0))) return rc;
-            k_cast<double, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, zd);
-        } else {
-            k_mul<<<vb, kVecThreads, 0, s>>>(n0, r, invd, zd);
-        }
-        HIPTRY(hipGetLastError());
-        return download_natural(zd, host_z);
-    }
-    int apply_operator(const double* host_x, double* host_y) override {
-        int rc;
-        if ((rc = upload_split<double, double>(g0, g0.nz, host_x, p))) return rc;
-        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1);
-        HIPTRY(hipGetLastError());
-        if ((rc = launch_apply())) return rc;
-        return download_natural(Ap, host_y);
-    }
-    int time_kernel(int which, int reps, double* avg_ms, double* bytes) override {
-        if (reps < 1) return fail(TH_ERR_ARG, "reps must be >= 1");
-        if (which != TH_K_APPLY && which != TH_K_COPY && !mg())
-            return fail(TH_ERR_ARG, "multigrid kernels need a multigrid solver");
-        if (which == TH_K_RESTRICT && L.size() < 2) return fail(TH_ERR_ARG, "no coarse level");
-        const double N = (double)g0.nx * g0.ny * g0.nz, sz = (double)sizeof(T);
-        auto launch = [&]() -> int {
-            switch (which) {
-                case TH_K_SWEEP:
-                    return launch_sweep<SW_PLAIN>(0, 0);
-                case TH_K_APPLY: {
-                    return launch_apply_kernel();
-                }
-                case TH_K_RESTRICT:
-                    return launch_restrict(0);
-                case TH_K_COPY:
-                    k_cast<double, double><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, p, Ap);
-                    HIPTRY(hipGetLastError());
-                    return TH_OK;
-                case TH_K_VCYCLE:
-                    return vcycle(0);
-                default:
-                    return fail(TH_ERR_ARG, "unknown kernel id");
-            }
-        };
-        switch (which) {
-            case TH_K_SWEEP: *bytes = 4.5 * sz * N; break;                  // b, x(other), gE, gN, gU, 1/pivot; x
-            case TH_K_APPLY: *bytes = (16.0 + 3.0 * sz) * N; break;         // p, Ap; gE, gN, gU
-            case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN; residual form: b, x, gE, gN, gU; coarse b
-                *bytes = (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
-                break;
-            case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
-            default: *bytes = 0.0;
-        }
-        k_reopen<<<1, 1, 0, s>>>(st);  // kernels return at once on a finished solve
-        HIPTRY(hipGetLastError());
-        int rc;
-        if ((rc = launch())) return rc;
-        hipEvent_t e0, e1;
-        HIPTRY(hipEventCreate(&e0));
-        HIPTRY(hipEventCreate(&e1));
-        hipError_t e = hipEventRecord(e0, s);
-        for (int k = 0; k < reps && e == hipSuccess; ++k)
-            if ((rc = launch())) break;
-        if (e == hipSuccess && !rc) e = hipEventRecord(e1, s);
-        if (e == hipSuccess && !rc) e = hipEventSynchronize(e1);
-        float ms = 0.0f;
-        if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("kernel timing: ") + hipGetErrorString(e));
-        *avg_ms = ms / reps;
-        return TH_OK;
-    }
-
-    int download_natural(const double* dev, double* host) {
-        k_to_natural<<<lin_blocks(nnat), 256, 0, s>>>(g0, dev, xnat);
-        HIPTRY(hipGetLastError());
-        HIPTRY(hipMemcpyAsync(host, xnat, nnat * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPTRY(hipStreamSynchronize(s));
-        return TH_OK;
-    }
-};
-
-}  // namespace th
+// solver.hip - the C ABI of the solver (include/thermal_hip.h) over
+// th::Solver (solver.h).
+#include "solver.h"
 
 // ===========================================================================
 // C ABI: solver

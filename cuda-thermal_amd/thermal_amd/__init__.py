@@ -127,6 +127,14 @@ SIGNATURES = {
     "th_transient_default_opts": (None, [C.POINTER(TransientOpts)]),
     "th_run_transient": (C.c_int, [_vp, C.POINTER(TransientOpts), C.POINTER(StepRecord), C.c_int, _ip, _dp,
                                    _ip, _dp]),
+    "th_batch_create": (C.c_int, [C.c_int, C.POINTER(C.POINTER(ModelDesc)), C.POINTER(SolverOpts), C.POINTER(_vp)]),
+    "th_batch_destroy": (None, [_vp]),
+    "th_batch_count": (C.c_int, [_vp]),
+    "th_batch_set_power": (C.c_int, [_vp, C.c_int, _dp]),
+    "th_batch_solve": (C.c_int, [_vp, C.c_double, C.c_int, _ip, _dp, _ip, _dp]),
+    "th_batch_get_x": (C.c_int, [_vp, C.c_int, _dp]),
+    "th_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
+    "th_batch_shape": (C.c_int, [_vp, _ip, _ip]),
     "th_surrogate_param_count": (C.c_long, []),
     "th_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
     "th_surrogate_destroy": (None, [_vp]),
@@ -561,3 +569,62 @@ class Surrogate:
         ms, fl = C.c_double(), C.c_double()
         _check(lib().th_surrogate_time_forward(self.h, batch, reps, C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
+
+
+class Batch:
+    """Batched steady solves (th_batch_*): many stacks, one launch per stage."""
+
+    def __init__(self, models, graphs=True, device=0):
+        descs, self._keep = [], []
+        for m in models:
+            if isinstance(m, Model):
+                d, k = m.desc(), m
+            else:
+                d, k = desc_from_arrays(m)
+            descs.append(d)
+            self._keep.append((d, k))
+        self.sizes = [(d.nx, d.ny, d.nz) for d in descs]
+        arr = (C.POINTER(ModelDesc) * len(descs))(*[C.pointer(d) for d in descs])
+        opts = make_opts("mg", "mixed", "flexible", graphs, device)
+        h = _vp()
+        self.h = None
+        _check(lib().th_batch_create(len(descs), arr, C.byref(opts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().th_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def set_power(self, i, P):
+        P = np.ascontiguousarray(P, np.float64)
+        _check(lib().th_batch_set_power(self.h, i, _d(P)))
+
+    def solve(self, tol=1e-10, maxit=500):
+        n = len(self)
+        it, rr, cv = np.zeros(n, np.int32), np.zeros(n), np.zeros(n, np.int32)
+        ms = C.c_double()
+        _check(lib().th_batch_solve(self.h, tol, maxit, it.ctypes.data_as(_ip), _d(rr), cv.ctypes.data_as(_ip),
+                                    C.byref(ms)))
+        return dict(iters=it, relres=rr, converged=cv.astype(bool), ms=ms.value)
+
+    def get_x(self, i):
+        nx, ny, nz = self.sizes[i]
+        x = np.empty(nx * ny * nz)
+        _check(lib().th_batch_get_x(self.h, i, _d(x)))
+        return x
+
+    def autotune(self, reps=5):
+        r, v, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().th_batch_autotune(self.h, reps, C.byref(r), C.byref(v), C.byref(ms)))
+        return dict(rows=r.value, vec_threads=v.value, vcycle_ms=ms.value)
+
+    def shape(self):
+        r, v = C.c_int(), C.c_int()
+        _check(lib().th_batch_shape(self.h, C.byref(r), C.byref(v)))
+        return r.value, v.value

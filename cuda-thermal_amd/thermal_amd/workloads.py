@@ -111,6 +111,23 @@ def config2_spec(seed=1000, tsv=True, cluster_count=205, active=(True, True)):
     return dict(layers=layers, blocks=blocks, uniform=uniform, tsvs=tsvs, h_top=2e4, h_bot=0.0, Lx=DIE, Ly=DIE)
 
 
+def config3_instances(seed=3000, count=64, sizes=(128, 256, 512), p_on=0.6):
+    """The batched workload: `count` (n, spec) pairs on the 10-plane default
+    stack, n drawn from `sizes` with equal probability; each of the 12
+    blocks per die of the 1024^2 TSV stack's floorplan (seed 1000, with its
+    TSV clusters and 2.5 W background per die) switched on with probability
+    p_on. All draws from one splitmix64 stream."""
+    rng = SplitMix64(seed)
+    base = config2_spec(1000)
+    out = []
+    for _ in range(count):
+        n = sizes[rng.below(len(sizes))]
+        blocks = [b for b in base["blocks"] if rng.uniform() < p_on]
+        spec = dict(base, layers=config0_layers(), blocks=blocks)
+        out.append((n, spec))
+    return out
+
+
 def config0_spec(hotspot_at=(5e-3, 5e-3), uniform_only=False):
     """Config 0: the default stack, tier 2 with 22 W uniform + 3 W in a
     centred 1 mm square, tier 1 with 25 W uniform (or 25 + 25 W uniform)."""

@@ -203,6 +203,28 @@ TH_API int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_re
                             int rec_cap, int* nsteps, double* T_final, int* all_converged,
                             double* solve_ms);
 
+/* ---- batched steady solves (no reference counterpart) ---------------------
+ * Many independent stacks (grid sizes may differ; plane counts <= 16 in one
+ * bucket of 4/8/12/16) solved from T = 0 together: every stage of a PCG
+ * iteration is one launch over all instances, each instance masked by its
+ * own convergence state. Multigrid, mixed precision, flexible beta and the
+ * lateral restriction only. The descriptors are copied at creation. */
+typedef struct th_batch th_batch;
+TH_API int th_batch_create(int count, const th_model_desc* const* models, const th_solver_opts* o, th_batch** out);
+TH_API void th_batch_destroy(th_batch* b);
+TH_API int th_batch_count(th_batch* b);
+TH_API int th_batch_set_power(th_batch* b, int i, const double* P);
+/* Per-instance iterations, recurrence rel-res and convergence (arrays of
+ * count); ms: wall time of the whole batch. */
+TH_API int th_batch_solve(th_batch* b, double tol, int maxit, int* iters, double* relres, int* converged,
+                          double* ms);
+TH_API int th_batch_get_x(th_batch* b, int i, double* x);
+/* Autotuner: times the V-cycle with 1, 2, 4 or 8 rows of 64 columns per
+ * block and the vector stages with 256- or 1024-thread blocks on this batch,
+ * keeps the fastest; reports the choice and the V-cycle time. */
+TH_API int th_batch_autotune(th_batch* b, int reps, int* rows, int* vec_threads, double* vcycle_ms);
+TH_API int th_batch_shape(th_batch* b, int* rows, int* vec_threads);
+
 /* ---- learned warm start (no reference counterpart) ------------------------
  * A U-Net surrogate (BF16 MFMA kernels, FP32 accumulation) mapping, on an
  * H x W grid over the die, 3 input features per cell (power density of die

@@ -1,0 +1,88 @@
+"""GPU tests of the batched solver: per-instance results equal to the
+single-instance solver's (same kernel bodies, same reductions), convergence
+masks, the autotuner, and a small instance of the batched workload."""
+import numpy as np
+import pytest
+
+import oracle as O
+import thermal_amd as T
+from thermal_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def hotspot(n, ny=None, at=(3e-3, 3e-3)):
+    m = T.Model.default(n, ny or n)
+    m.add_uniform_power(0, 22.0)
+    m.add_hotspot(0, at[0], at[1], 1e-3, 3.0)
+    m.add_uniform_power(2, 25.0)
+    return m.finalize()
+
+
+def mixed_models():
+    return [T.Model.example_case(37, 23), hotspot(64), hotspot(48, 40, at=(5e-3, 5e-3)),
+            T.Model.example_case(16, 12), T.Model.example_case(100, 60)]
+
+
+def single(m, tol):
+    s = T.Solver(m)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(tol, 200)
+    return st, s.get_x()
+
+
+def test_batch_equals_single_solves():
+    """Instances of different sizes (one small enough that its whole
+    V-cycle is the tail) converge in the same iterations to the same bits
+    as separate solves."""
+    ms = mixed_models()
+    b = T.Batch(ms)
+    r = b.solve(1e-10, 200)
+    assert r["converged"].all()
+    for i, m in enumerate(ms):
+        st, x = single(m, 1e-10)
+        assert r["iters"][i] == st["iters"], i
+        assert np.array_equal(b.get_x(i), x), i
+
+
+def test_convergence_masks():
+    """A zero-power instance finishes at the start (x = 0) while the others
+    iterate; their results are unaffected."""
+    ms = mixed_models()[:3]
+    b = T.Batch(ms)
+    b.set_power(1, np.zeros(64 * 64 * 10))
+    r = b.solve(1e-10, 200)
+    assert r["converged"].all() and r["iters"][1] == 0
+    assert not b.get_x(1).any()
+    st, x = single(ms[0], 1e-10)
+    assert r["iters"][0] == st["iters"] and np.array_equal(b.get_x(0), x)
+
+
+def test_autotuned_shapes_give_the_same_answers():
+    ms = mixed_models()
+    b = T.Batch(ms)
+    tune = b.autotune(3)
+    assert tune["rows"] in (1, 2, 4, 8) and tune["vec_threads"] in (256, 1024)
+    assert b.shape() == (tune["rows"], tune["vec_threads"])
+    r = b.solve(1e-10, 200)
+    for i, m in enumerate(ms):
+        st, x = single(m, 1e-10)
+        assert abs(int(r["iters"][i]) - st["iters"]) <= 1
+        assert np.abs(b.get_x(i) - x).max() <= 1e-8 * np.abs(x).max()
+
+
+def test_small_batched_workload_vs_oracle():
+    """Eight instances of the batched workload's family on small grids: each
+    converges, and matches the oracle's FP64 answer."""
+    inst = W.config3_instances(seed=42, count=8, sizes=(24, 32, 40))
+    models = [W.build(spec, n) for n, spec in inst]
+    b = T.Batch(models)
+    r = b.solve(1e-10, 200)
+    assert r["converged"].all()
+    for i, m in enumerate(models):
+        a = m.arrays()
+        ref = O.steady(a, tol=1e-11, mg_double=True)
+        x = b.get_x(i)
+        assert np.abs(x - ref["T"]).max() < 5e-4
+        assert O.true_relres(a, x) < 1e-9
