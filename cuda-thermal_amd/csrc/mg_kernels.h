@@ -24,64 +24,56 @@ enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 constexpr int kTailThreads = 512;
 constexpr int kMaxTail = 8;
 constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
-constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail
-constexpr int kTailLdsThreads = 512;            // 256 VGPRs per lane: no spills for nz <= 16
+constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 4 KB static)
+constexpr int kTailLdsThreads = 1024;           // plane-parallel sweeps: 64 / LP columns per wave
 
-// Exact solve of column (i, j) with its lateral neighbours fixed
-// (mg3d.h:67-84), in place in x. MODE: see above. xc/gc: the coarse level
-// (SW_PROLONG only).
-template <class T, int NZ, int MODE>
-__device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__ gE, const T* __restrict__ gN,
-                                             const T* __restrict__ gU, const T* __restrict__ finv,
-                                             const T* __restrict__ b, T* __restrict__ x, int i, int j,
-                                             const Geo& gc, const T* __restrict__ xc) {
-    const Col c = make_col(g, i, j);
-    int qw = 0, qe = 0, qs = 0, qn = 0;
-    if (MODE == SW_PROLONG) {  // coarse parents of the 4 neighbours
-        qw = gc.pos((c.hw ? i - 1 : i) >> 1, j >> 1);
-        qe = gc.pos((c.he ? i + 1 : i) >> 1, j >> 1);
-        qs = gc.pos(i >> 1, (c.hs ? j - 1 : j) >> 1);
-        qn = gc.pos(i >> 1, (c.hn ? j + 1 : j) >> 1);
-    }
-    // Pass 1: every plane's right-hand side with the lateral neighbours held
-    // fixed. No plane depends on another, so all loads are in flight at once,
-    // which is what hides memory latency on the small coarse levels, where a
-    // SIMD holds one or two waves.
-    T s[NZ], inv[NZ], gu[NZ];
-#pragma unroll
-    for (int l = 0; l < NZ; ++l) {
-        if (l < g.nz) {
-            const long base = (long)l * g.plane;
-            const long id = base + c.c0;
-            T v = b[id];
-            if (MODE != SW_ZERO) {
-                T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
-                if (MODE == SW_PROLONG) {
-                    const long cb = (long)l * gc.plane;
-                    xw = xw + xc[cb + qw];
-                    xe = xe + xc[cb + qe];
-                    xsv = xsv + xc[cb + qs];
-                    xn = xn + xc[cb + qn];
-                }
-                const T Gw = gE[base + c.pw], Ge = gE[id], Gs = gN[base + c.ps], Gn = gN[id];
-                T ls = 0;
-                if (c.hw) ls += Gw * xw;
-                if (c.he) ls += Ge * xe;
-                if (c.hs) ls += Gs * xsv;
-                if (c.hn) ls += Gn * xn;
-                v = v + ls;
-            }
-            s[l] = v;
-            inv[l] = finv[id];
-            gu[l] = gU[id];
+// Coarse parents of the 4 lateral neighbours of column (i, j) (SW_PROLONG).
+struct Parents {
+    int w, e, s, n;
+};
+__device__ __forceinline__ Parents parents(const Geo& gc, const Col& c, int i, int j) {
+    return Parents{gc.pos((c.hw ? i - 1 : i) >> 1, j >> 1), gc.pos((c.he ? i + 1 : i) >> 1, j >> 1),
+                   gc.pos(i >> 1, (c.hs ? j - 1 : j) >> 1), gc.pos(i >> 1, (c.hn ? j + 1 : j) >> 1)};
+}
+
+// Right-hand side of plane l of column c with its lateral neighbours held
+// fixed: b + sum G x_nb (SW_PROLONG: each neighbour plus its coarse parent's
+// correction; SW_ZERO: neighbours taken as 0).
+template <class T, int MODE>
+__device__ __forceinline__ T column_rhs(const Geo& g, const T* __restrict__ gE, const T* __restrict__ gN,
+                                        const T* __restrict__ b, const T* __restrict__ x, const Col& c, int l,
+                                        const Geo& gc, const T* __restrict__ xc, const Parents& pa) {
+    const long base = (long)l * g.plane;
+    const long id = base + c.c0;
+    T v = b[id];
+    if (MODE != SW_ZERO) {
+        T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
+        if (MODE == SW_PROLONG) {
+            const long cb = (long)l * gc.plane;
+            xw = xw + xc[cb + pa.w];
+            xe = xe + xc[cb + pa.e];
+            xsv = xsv + xc[cb + pa.s];
+            xn = xn + xc[cb + pa.n];
         }
+        const T Gw = gE[base + c.pw], Ge = gE[id], Gs = gN[base + c.ps], Gn = gN[id];
+        T ls = 0;
+        if (c.hw) ls += Gw * xw;
+        if (c.he) ls += Ge * xe;
+        if (c.hs) ls += Gs * xsv;
+        if (c.hn) ls += Gn * xn;
+        v = v + ls;
     }
-    // Pass 2: Thomas forward elimination and back substitution in registers,
-    // upper factor rebuilt as (-gU) * (1/pivot).
+    return v;
+}
+
+// Thomas forward elimination and back substitution of one column in place
+// in s (planes 0..nz-1), the upper factor rebuilt as (-gU) * (1/pivot).
+template <class T, int NZ>
+__device__ __forceinline__ void column_thomas(int nz, T (&s)[NZ], const T (&inv)[NZ], const T (&gu)[NZ]) {
     T prev = 0;
 #pragma unroll
     for (int l = 0; l < NZ; ++l) {
-        if (l < g.nz) {
+        if (l < nz) {
             const T lower = (l > 0) ? -gu[l - 1] : T(0);
             prev = (s[l] - lower * prev) * inv[l];
             s[l] = prev;
@@ -89,10 +81,93 @@ __device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__
     }
 #pragma unroll
     for (int l = NZ - 2; l >= 0; --l)
-        if (l < g.nz - 1) s[l] = s[l] - ((-gu[l]) * inv[l]) * s[l + 1];
+        if (l < nz - 1) s[l] = s[l] - ((-gu[l]) * inv[l]) * s[l + 1];
+}
+
+// Exact solve of column (i, j) with its lateral neighbours fixed
+// (mg3d.h:67-84), in place in x. MODE: see above. xc/gc: the coarse level
+// (SW_PROLONG only). Every plane's right-hand side is gathered first (no
+// plane depends on another, so all loads are in flight at once), then the
+// recurrence runs in registers.
+template <class T, int NZ, int MODE>
+__device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__ gE, const T* __restrict__ gN,
+                                             const T* __restrict__ gU, const T* __restrict__ finv,
+                                             const T* __restrict__ b, T* __restrict__ x, int i, int j,
+                                             const Geo& gc, const T* __restrict__ xc) {
+    const int nz = g.nz;
+    const Col c = make_col(g, i, j);
+    const Parents pa = MODE == SW_PROLONG ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
+    T s[NZ], inv[NZ], gu[NZ];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < nz) {
+            const long id = (long)l * g.plane + c.c0;
+            s[l] = column_rhs<T, MODE>(g, gE, gN, b, x, c, l, gc, xc, pa);
+            inv[l] = finv[id];
+            gu[l] = gU[id];
+        }
+    }
+    column_thomas<T, NZ>(nz, s, inv, gu);
 #pragma unroll
     for (int l = 0; l < NZ; ++l)
-        if (l < g.nz) x[(long)l * g.plane + c.c0] = s[l];
+        if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
+}
+
+// Plane-parallel sweep for levels with few columns, where the column walk
+// above is latency-bound (one thread issuing ~11 loads per plane): LP lanes
+// per column, one per plane, 256 / LP columns per 256-thread block (a column
+// never spans two wavefronts). Each lane gathers its plane's right-hand
+// side, 1/pivot and gU into LDS; the column's first lane runs the Thomas
+// recurrence from LDS (the same arithmetic in the same order as
+// sweep_column) and every lane writes its plane back.
+constexpr int kPPThreads = 256;
+template <class T, int LP, int MODE>
+__device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const Ops<T>& o, const T* __restrict__ finv,
+                                              const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
+                                              const T* __restrict__ xc, State* st,
+                                              const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
+    __shared__ T sh_s[kPPThreads], sh_i[kPPThreads], sh_u[kPPThreads];
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
+    constexpr int CPB = kPPThreads / LP;
+    const int tid = threadIdx.x, l = tid % LP, cb = tid / LP;
+    const long q = (long)B.x * CPB + cb;  // column of this colour: t + hx * j
+    const int j = (int)(q / g.hx), t = (int)(q - (long)j * g.hx);
+    const int i = 2 * t + ((j + colour) & 1);
+    const bool colok = j < g.ny && i < g.nx;
+    const bool ok = colok && l < g.nz;
+    Col c{};
+    if (colok) c = make_col(g, i, j);
+    if (ok) {
+        const Parents pa = MODE == SW_PROLONG ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
+        const long id = (long)l * g.plane + c.c0;
+        sh_s[tid] = column_rhs<T, MODE>(g, o.gE, o.gN, b, x, c, l, gc, xc, pa);
+        sh_i[tid] = finv[id];
+        sh_u[tid] = o.gU[id];
+    }
+    __syncthreads();
+    if (colok && l == 0) {
+        T s[LP], inv[LP], gu[LP];
+#pragma unroll
+        for (int k = 0; k < LP; ++k) {
+            s[k] = sh_s[tid + k];
+            inv[k] = sh_i[tid + k];
+            gu[k] = sh_u[tid + k];
+        }
+        column_thomas<T, LP>(g.nz, s, inv, gu);
+#pragma unroll
+        for (int k = 0; k < LP; ++k) sh_s[tid + k] = s[k];
+    }
+    __syncthreads();
+    if (ok) x[(long)l * g.plane + c.c0] = sh_s[tid];
+}
+template <class T, int LP, int MODE>
+__global__ __launch_bounds__(kPPThreads) void k_sweep_pp(Geo g, Ops<T> o, const T* __restrict__ finv,
+                                                          const T* __restrict__ b, T* __restrict__ x, int colour,
+                                                          Geo gc, const T* __restrict__ xc, State* st,
+                                                          const double* __restrict__ cons_part, int cons_nblk,
+                                                          int cons_mode) {
+    sweep_pp_body<T, LP, MODE>(hw_blk(), g, o, finv, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 // Coarse b at (I, J, l): sum over the (up to 4) children, row-major, of the
@@ -314,46 +389,116 @@ inline size_t tail_lds_bytes(const Geo* g, int n) {
     return b;
 }
 
-// The same tail with every level copied into LDS first (FP32, fast
-// restriction): the 20-odd barrier-separated phases then wait on LDS
-// (~100-cycle) instead of L2 latency. Only the top level's x goes back to
-// global memory, for the prolongation into the level above.
-template <int NZ>
-__device__ __forceinline__ void tail_lds_body(const Tail<float>& t, State* st, const double* __restrict__ cons_part,
-                                              int cons_nblk, int cons_mode, float* sm, TailLevel<float>* S) {
-    if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
+// The same tail with every level copied into LDS first (FP32, lateral
+// restriction), every sweep plane-parallel: LP lanes per column gather the
+// planes' right-hand sides at once, the column's first lane runs the Thomas
+// recurrence (sweep_column's arithmetic), every lane writes its plane back.
+// Level arrays are addressed from the dynamic LDS base, so every access is
+// an LDS access. Only the top level's x goes back to global memory, for the
+// prolongation into the level above.
+struct LdsLevel {
+    Geo g;
+    float *gE, *gN, *gU, *fi, *x, *b;
+};
+__device__ __forceinline__ LdsLevel lds_level(const Tail<float>& t, float* sm, int k) {
     size_t off = 0;
-    for (int k = 0; k < t.n; ++k) {
-        const TailLevel<float>& G = t.L[k];
-        const long cells = G.g.cells();
-        float* gE = sm + off;
-        float* gN = gE + cells;
-        float* gU = gN + cells;
-        float* fi = gU + cells;
-        float* x = fi + cells;
-        float* b = x + cells;
-        off += 6 * (size_t)cells;
-        for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
-            gE[q] = G.o.gE[q];
-            gN[q] = G.o.gN[q];
-            gU[q] = G.o.gU[q];
-            fi[q] = G.finv[q];
-            if (k == 0) b[q] = G.b[q];
+    for (int m = 0; m < k; ++m) off += 6 * (size_t)t.L[m].g.cells();
+    const Geo g = t.L[k].g;
+    const long cells = g.cells();
+    float* p = sm + off;
+    return LdsLevel{g, p, p + cells, p + 2 * cells, p + 3 * cells, p + 4 * cells, p + 5 * cells};
+}
+
+template <int LP, int MODE>
+__device__ __forceinline__ void tail_sweep_pp(const LdsLevel& F, int colour, const LdsLevel& C, float* scr_s) {
+    constexpr int CPP = kTailLdsThreads / LP;  // columns per pass
+    const int tid = threadIdx.x, l = tid % LP;
+    const Geo& g = F.g;
+    const int ncols = g.hx * g.ny;
+    for (int q0 = 0; q0 < ncols; q0 += CPP) {
+        const int q = q0 + tid / LP;
+        const int j = q / g.hx, t = q - j * g.hx;
+        const int i = 2 * t + ((j + colour) & 1);
+        const bool colok = q < ncols && i < g.nx;
+        const bool ok = colok && l < g.nz;
+        Col c{};
+        if (colok) c = make_col(g, i, j);
+        if (ok) {
+            const Parents pa = MODE == SW_PROLONG ? parents(C.g, c, i, j) : Parents{0, 0, 0, 0};
+            scr_s[tid] = column_rhs<float, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
         }
-        if (threadIdx.x == 0) S[k] = TailLevel<float>{G.g, Ops<float>{gE, gN, gU, nullptr, nullptr}, fi, x, b};
+        __syncthreads();
+        if (colok && l == 0) {
+            float sv[LP], iv[LP], uv[LP];
+#pragma unroll
+            for (int k = 0; k < LP; ++k) {
+                sv[k] = scr_s[tid + k];
+                if (k < g.nz) {
+                    const long id = (long)k * g.plane + c.c0;
+                    iv[k] = F.fi[id];
+                    uv[k] = F.gU[id];
+                }
+            }
+            column_thomas<float, LP>(g.nz, sv, iv, uv);
+#pragma unroll
+            for (int k = 0; k < LP; ++k) scr_s[tid + k] = sv[k];
+        }
+        __syncthreads();
+        if (ok) F.x[(long)l * g.plane + c.c0] = scr_s[tid];
     }
     __syncthreads();
-    tail_vcycle<float, NZ, kTailLdsThreads>(S, t.n, 0.0f, 1);
+}
+
+template <int NZ>
+__device__ __forceinline__ void tail_lds_body(const Tail<float>& t, State* st, const double* __restrict__ cons_part,
+                                              int cons_nblk, int cons_mode, float* sm) {
+    constexpr int LP = NZ <= 4 ? 4 : NZ <= 8 ? 8 : 16;
+    __shared__ float scr_s[kTailLdsThreads];
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
+    for (int k = 0; k < t.n; ++k) {
+        const TailLevel<float>& G = t.L[k];
+        const LdsLevel D = lds_level(t, sm, k);
+        const long cells = G.g.cells();
+        for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
+            D.gE[q] = G.o.gE[q];
+            D.gN[q] = G.o.gN[q];
+            D.gU[q] = G.o.gU[q];
+            D.fi[q] = G.finv[q];
+            if (k == 0) D.b[q] = G.b[q];
+        }
+    }
+    __syncthreads();
+    const int n = t.n;
+    for (int k = 0; k + 1 < n; ++k) {
+        const LdsLevel F = lds_level(t, sm, k), C = lds_level(t, sm, k + 1);
+        tail_sweep_pp<LP, SW_ZERO>(F, 0, C, scr_s);
+        tail_sweep_pp<LP, SW_PLAIN>(F, 1, C, scr_s);
+        const int cells = C.g.nx * C.g.ny * C.g.nz;
+        for (int q = threadIdx.x; q < cells; q += kTailLdsThreads) {
+            const int I = q % C.g.nx, rest = q / C.g.nx;
+            restrict_cell_fast<float>(F.g, F.gE, F.gN, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny);
+        }
+        __syncthreads();
+    }
+    {
+        const LdsLevel N = lds_level(t, sm, n - 1);
+        tail_sweep_pp<LP, SW_ZERO>(N, 0, N, scr_s);
+    }
+    for (int k = n - 2; k >= 0; --k) {
+        const LdsLevel F = lds_level(t, sm, k), C = lds_level(t, sm, k + 1);
+        tail_sweep_pp<LP, SW_PROLONG>(F, 1, C, scr_s);
+        tail_sweep_pp<LP, SW_PLAIN>(F, 0, C, scr_s);
+    }
+    const LdsLevel T0 = lds_level(t, sm, 0);
     const long cells0 = t.L[0].g.cells();
-    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = S[0].x[q];
+    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = T0.x[q];
 }
 template <int NZ>
 __global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, State* st, const double* __restrict__ cons_part,
                                                    int cons_nblk, int cons_mode) {
     extern __shared__ float sm[];
-    __shared__ TailLevel<float> S[kMaxTail];  // level views into LDS (not in scratch)
-    tail_lds_body<NZ>(t, st, cons_part, cons_nblk, cons_mode, sm, S);
+    tail_lds_body<NZ>(t, st, cons_part, cons_nblk, cons_mode, sm);
 }
 
 }  // namespace th

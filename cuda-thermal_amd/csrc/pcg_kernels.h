@@ -79,6 +79,7 @@ template <class T, int NZ>
 __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops<T>& o, const double* __restrict__ p,
                                            double* __restrict__ Ap, const State* __restrict__ st,
                                            double* __restrict__ part) {
+    const int nz = g.nz;
     if (st->done) return;
     const int q = B.x * blockDim.x + threadIdx.x;
     const int j = B.y * blockDim.y + threadIdx.y;
@@ -94,9 +95,9 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
         double pb = 0.0, pc = p[c.c0], gdn = (double)o.gB[c.c0];
 #pragma unroll
         for (int l = 0; l < NZ; ++l) {
-            if (l < g.nz) {
+            if (l < nz) {
                 const long base = (long)l * g.plane, id = base + c.c0;
-                const double pu = (l < g.nz - 1) ? p[id + g.plane] : 0.0;
+                const double pu = (l < nz - 1) ? p[id + g.plane] : 0.0;
                 const double gu = (double)gU[id];
                 const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id];
                 const double Gs = (double)gN[base + c.ps], Gn = (double)gN[id];
@@ -114,7 +115,7 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
                 if (c.hs) s += Gs * xs;
                 if (c.hn) s += Gn * xn;
                 if (l > 0) s += gdn * pb;
-                if (l < g.nz - 1) s += gu * pu;
+                if (l < nz - 1) s += gu * pu;
                 const double a = d * pc - s;
                 Ap[id] = a;
                 v[0] += pc * a;
