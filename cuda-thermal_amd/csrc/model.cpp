@@ -159,6 +159,24 @@ void power_block(Model& m, int layer, double x0, double y0, double w, double h, 
     }
 }
 
+// Max and area-weighted mean of T over a rectangle of a layer's planes
+// (main3d.cu:286-301).
+void block_temps(const Model& m, const double* T, int layer, double x0, double y0, double w, double h, double& tmax,
+                 double& tavg) {
+    tmax = -1e30;
+    double sum = 0, wsum = 0;
+    for (int l = 0; l < m.nz; ++l) {
+        if (m.plane_layer[l] != layer) continue;
+        covered_cells(m, x0, y0, w, h, [&](size_t c, double frac) {
+            const double t = T[(size_t)l * m.nxy() + c];
+            tmax = std::max(tmax, t);
+            sum += frac * t;
+            wsum += frac;
+        });
+    }
+    tavg = wsum > 0 ? sum / wsum : 0.0;
+}
+
 void tsv_region(Model& m, int layer, double x0, double y0, double w, double h, double f) {
     for (int l = 0; l < m.nz; ++l) {
         if (m.plane_layer[l] != layer) continue;
@@ -476,6 +494,8 @@ int th_model_load_case(const char* dir, int nx, int ny, int with_tsv, th_model**
         delete h;
         return rc;
     }
+    h->blocks = c.blocks;
+    h->tsv_regions = with_tsv ? (int)c.tsvs.size() : 0;
     *out = h;
     return TH_OK;
 }
@@ -484,12 +504,44 @@ int th_model_example_case(int nx, int ny, int with_tsv, th_model** out) {
     clear_error();
     if (!out) return fail(TH_ERR_ARG, "null out");
     th_model* h = new th_model();
-    const int rc = case_model(builtin_example(), nx, ny, with_tsv != 0, h->m);
+    const Case c = builtin_example();
+    const int rc = case_model(c, nx, ny, with_tsv != 0, h->m);
     if (rc) {
         delete h;
         return rc;
     }
+    h->blocks = c.blocks;
+    h->tsv_regions = with_tsv ? (int)c.tsvs.size() : 0;
     *out = h;
+    return TH_OK;
+}
+
+int th_model_block_count(const th_model* m) { return m ? (int)m->blocks.size() : -1; }
+int th_model_tsv_count(const th_model* m) { return m ? m->tsv_regions : -1; }
+
+int th_model_block_info(const th_model* m, int k, const char** name, int* layer, double* x0, double* y0, double* w,
+                        double* h, double* watts) {
+    clear_error();
+    if (!m || k < 0 || k >= (int)m->blocks.size()) return fail(TH_ERR_ARG, "no such block");
+    const Block& b = m->blocks[k];
+    if (name) *name = b.name.c_str();
+    if (layer) *layer = b.layer;
+    if (x0) *x0 = b.x0;
+    if (y0) *y0 = b.y0;
+    if (w) *w = b.w;
+    if (h) *h = b.h;
+    if (watts) *watts = b.watts;
+    return TH_OK;
+}
+
+int th_model_block_temps(const th_model* m, int k, const double* T, double* tmax, double* tavg) {
+    clear_error();
+    if (!m || !T || k < 0 || k >= (int)m->blocks.size()) return fail(TH_ERR_ARG, "bad argument");
+    const Block& b = m->blocks[k];
+    double a = 0, v = 0;
+    block_temps(m->m, T, b.layer, b.x0, b.y0, b.w, b.h, a, v);
+    if (tmax) *tmax = a;
+    if (tavg) *tavg = v;
     return TH_OK;
 }
 

@@ -53,6 +53,8 @@ bool hotspot(Model& m, int layer, double xc, double yc, double size, double watt
 double total_power(const Model& m);
 std::vector<double> chain_1d(const Model& m);
 double heat_out(const Model& m, const double* T);
+void block_temps(const Model& m, const double* T, int layer, double x0, double y0, double w, double h, double& tmax,
+                 double& tavg);
 
 // A case folder (case3d.h:1-13 format).
 struct Block {
@@ -78,4 +80,6 @@ int case_model(const Case& c, int nx, int ny, bool with_tsv, Model& m);
 
 struct th_model {
     th::Model m;
+    std::vector<th::Block> blocks;  // power blocks of the case it was built from
+    int tsv_regions = 0;
 };

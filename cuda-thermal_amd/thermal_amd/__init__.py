@@ -104,6 +104,10 @@ SIGNATURES = {
     "th_model_analytic_1d": (C.c_int, [_vp, _dp]),
     "th_model_heat_out": (C.c_double, [_vp, _dp]),
     "th_model_describe": (C.c_int, [_vp, C.POINTER(ModelDesc)]),
+    "th_model_block_count": (C.c_int, [_vp]),
+    "th_model_tsv_count": (C.c_int, [_vp]),
+    "th_model_block_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), _ip, _dp, _dp, _dp, _dp, _dp]),
+    "th_model_block_temps": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "th_solver_default_opts": (None, [C.POINTER(SolverOpts)]),
     "th_solver_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(SolverOpts), C.POINTER(_vp)]),
     "th_solver_destroy": (None, [_vp]),

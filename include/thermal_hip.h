@@ -79,6 +79,17 @@ TH_API double th_model_total_power(const th_model* m);                /* stack3d
 TH_API int th_model_analytic_1d(const th_model* m, double* T /* nz */); /* stack3d.h:279 */
 TH_API double th_model_heat_out(const th_model* m, const double* T);  /* stack3d.h:254 */
 
+/* Power blocks and TSV regions of the case a model was built from
+ * (th_model_load_case / th_model_example_case; CaseSpec, case3d.h:24-33);
+ * 0 for other models. */
+TH_API int th_model_block_count(const th_model* m);
+TH_API int th_model_tsv_count(const th_model* m);
+TH_API int th_model_block_info(const th_model* m, int k, const char** name, int* layer, double* x0,
+                               double* y0, double* w, double* h, double* watts);
+/* Max and area-weighted mean of T (natural layout) over block k's rectangle
+ * on its layer's planes (block_temps, main3d.cu:286-301). */
+TH_API int th_model_block_temps(const th_model* m, int k, const double* T, double* tmax, double* tavg);
+
 /* What the solver reads from a finalized Model3D (pcg3d_gpu.cu:372-391).
  * All arrays are copied by th_solver_create. */
 typedef struct {

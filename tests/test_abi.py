@@ -48,3 +48,26 @@ def test_errors_without_gpu():
     L.th_solver_destroy(None)  # never fails
     with pytest.raises(T.ThermalError):
         T.Model.load_case("/nonexistent/case", 8)
+
+
+def test_cli_reports_a_bad_case_folder(tmp_path):
+    """thermal3d exits 1 with the loader's message before touching the GPU."""
+    import subprocess
+
+    cli = os.path.join(ROOT, "cuda-thermal_amd", "bin", "thermal3d")
+    p = subprocess.run([cli, "case", str(tmp_path / "missing"), "16"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "Could not load case" in p.stdout and "stack.csv" in p.stdout
+    p = subprocess.run([cli, "export3dice", "a", "1", "b"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1
+
+
+def test_case_blocks_exported():
+    m = T.Model.load_case(os.path.join(ROOT, "tests", "golden", "cases", "example"), 16)
+    L = T.lib()
+    assert L.th_model_block_count(m.h) == 8 and L.th_model_tsv_count(m.h) == 4
+    name = C.c_char_p()
+    layer = C.c_int()
+    v = [C.c_double() for _ in range(5)]
+    assert L.th_model_block_info(m.h, 0, C.byref(name), C.byref(layer), *[C.byref(x) for x in v]) == 0
+    assert name.value and 0 <= layer.value < 5 and v[4].value > 0
+    assert L.th_model_block_info(m.h, 99, None, None, None, None, None, None, None) == 1
