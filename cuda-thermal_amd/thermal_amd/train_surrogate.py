@@ -26,6 +26,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def synthetic(sur, count, seed, chunk=64):
+    """Smooth random features and targets (no solver): for timing the
+    training kernels alone."""
+    import time
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    n = sur.H
+    k = np.exp(-0.5 * (np.fft.fftfreq(n)[:, None] ** 2 + np.fft.fftfreq(n)[None, :] ** 2) / 0.02 ** 2)
+    smooth = lambda a: np.real(np.fft.ifft2(np.fft.fft2(a, axes=(0, 1)) * k[..., None], axes=(0, 1)))  # noqa: E731
+    done = 0
+    while done < count:
+        b = min(chunk, count - done)
+        x = np.stack([np.abs(smooth(rng.standard_normal((n, n, 3)))) for _ in range(b)]).astype(np.float32)
+        t = np.stack([smooth(rng.standard_normal((n, n, Surrogate.N_OUT))) for _ in range(b)]).astype(np.float32)
+        sur.add(x, t)
+        done += b
+    return dict(samples=done, seconds=time.perf_counter() - t0, mean_iters=0.0)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=20000)
@@ -38,13 +58,15 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=S.WEIGHTS)
     ap.add_argument("--report", default=None, help="JSON summary path")
+    ap.add_argument("--synthetic", action="store_true",
+                    help="smooth random features and targets instead of solver data (kernel timing only)")
     a = ap.parse_args(argv)
 
     sur = Surrogate(S.GRID, S.GRID, a.seed)
     total = a.samples + a.val
     sur.reserve(total)
     log(f"surrogate: {Surrogate.param_count()} parameters; generating {total} samples")
-    gen = S.generate(sur, total, n_tsv=a.n_tsv, log=log)
+    gen = synthetic(sur, total, a.seed) if a.synthetic else S.generate(sur, total, n_tsv=a.n_tsv, log=log)
     log(f"  generated {gen['samples']} samples in {gen['seconds']:.0f} s")
     n_train = a.samples
     mse0 = sur.eval(n_train, a.val)
