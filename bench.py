@@ -107,15 +107,18 @@ def cpu_baseline(folder, n, iters):
 def transient_leg():
     """The 800x800x10 backward-Euler run from ambient: 100 steps, dt0 = 10 us
     growing 10% per step with no cap, each step warm-started and replayed
-    as graphs; per-step energy balance and wall time for the 100 steps."""
+    as graphs; per-step energy balance and wall time for the 100 steps, in
+    FP64, mixed, and mixed with each step started from the extrapolation of
+    the last two."""
     m = T.hotspot_model(800, 5e-3, 5e-3)
     P = m.total_power()
     kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
     out = {}
-    for prec in ("fp64", "mixed"):
-        s = T.Solver(m, precision=prec)
-        T.run_transient(s, **dict(kw, max_steps=3))  # warm-up
-        g = T.run_transient(s, **kw)
+    for prec in ("fp64", "mixed", "mixed+predictor"):
+        s = T.Solver(m, precision=prec.split("+")[0])
+        pred = prec.endswith("predictor")
+        T.run_transient(s, predictor=pred, **dict(kw, max_steps=3))  # warm-up
+        g = T.run_transient(s, predictor=pred, **kw)
         out[prec] = dict(ms=round(g["solve_ms"], 2), steps=len(g["t"]), t_end_s=float(g["t"][-1]),
                          iterations=int(g["iters"].sum()), converged=g["converged"],
                          energy_balance_max=float((np.abs(g["q_out"] + g["q_stored"] - P) / P).max()),
@@ -123,6 +126,7 @@ def transient_leg():
         out[prec + "_T"] = g["T"]
         s.close()
         log(f"  transient {prec}: {out[prec]}")
+    out["max_abs_err_predictor_vs_fp64_K"] = float(np.abs(out.pop("mixed+predictor_T") - out["fp64_T"]).max())
     out["max_abs_err_mixed_vs_fp64_K"] = float(np.abs(out.pop("mixed_T") - out.pop("fp64_T")).max())
     out["workload"] = "800x800x10 two-tier stack, 25 W + 25 W with a 3 W hotspot at (5,5) mm, 100 BE steps"
     return out

@@ -71,7 +71,7 @@ struct State {
     // Transient stepping on the device (transient3d.h:57-84): the time, the
     // step size of the current operator and of the next step, the stepping
     // rule, the heat content after the last step and that step's record.
-    double t, dt, dt_next, growth, dt_max, t_end, heat_prev;
+    double t, dt, dt_next, dt_prev, growth, dt_max, t_end, heat_prev;
     int refactor;  // 1: the level factors must be rebuilt for inv_dt
     int steps;
     StepRec last;

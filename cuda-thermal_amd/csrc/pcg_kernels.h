@@ -325,6 +325,7 @@ static __global__ void k_set_flexible(State* st, int f) { st->flexible = f; }
 static __global__ void k_step_init(State* st, double dt0, double growth, double dt_max, double t_end) {
     st->t = 0.0;
     st->dt = -1.0;
+    st->dt_prev = -1.0;
     st->dt_next = dt0;
     st->growth = growth;
     st->dt_max = dt_max;
@@ -337,6 +338,7 @@ static __global__ void k_step_init(State* st, double dt0, double growth, double 
 // Then tolerance and flags of the step's solve, as k_begin.
 static __global__ void k_step_begin(State* st, double tol, int maxit) {
     const double dt = fmin(st->dt_next, st->t_end - st->t);
+    st->dt_prev = st->dt;
     st->refactor = dt != st->dt;
     if (st->refactor) {
         st->dt = dt;
@@ -365,6 +367,21 @@ static __global__ void k_step_end(State* st) {
     st->heat_prev = st->e_heat;
     st->dt_next = fmin(dt * st->growth, st->dt_max);
     st->steps += 1;
+}
+
+// Predicted start of a backward-Euler step, after its right-hand side has
+// taken T_old from x: linear extrapolation of the last two steps,
+// x = T_old + (dt / dt_prev) (T_old - T_older); xprev keeps T_old. The first
+// step (no previous dt) starts from T_old.
+static __global__ __launch_bounds__(kVecThreads) void k_predict(long n, double* __restrict__ x, double* __restrict__ xprev,
+                                                         const State* __restrict__ st) {
+    const double r = st->dt_prev > 0.0 ? st->dt / st->dt_prev : 0.0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double xo = x[k];
+        x[k] = xo + r * (xo - xprev[k]);
+        xprev[k] = xo;
+    }
 }
 
 // x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).

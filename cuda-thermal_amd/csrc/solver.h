@@ -623,12 +623,17 @@ struct Solver final : SolverBase {
     //   tail: energy terms of the new x and the step's record.
     // The host keeps its own copy of t and dt (the same double arithmetic
     // as the device) to know when to stop, and waits once per step.
-    int enqueue_step_head(double tol, int maxit) {
+    double* xprev = nullptr;  // T_older for the step predictor (allocated on first use)
+    int enqueue_step_head(double tol, int maxit, int predictor) {
         k_step_begin<<<1, 1, 0, s>>>(st, tol, maxit);
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = enqueue_factors())) return rc;
         if ((rc = set_rhs_transient())) return rc;
+        if (predictor) {
+            k_predict<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, xprev, st);
+            HIPTRY(hipGetLastError());
+        }
         return enqueue_start();
     }
     int enqueue_step_tail() {
@@ -653,8 +658,9 @@ struct Solver final : SolverBase {
             if (gh) (void)hipGraphExecDestroy(gh);
             if (gt) (void)hipGraphExecDestroy(gt);
         };
+        if (o.predictor && !xprev && (rc = dalloc(&xprev, n0))) return rc;
         if (opt.use_graphs && mg()) {
-            if ((rc = capture_fn(&gh, [&]() { return enqueue_step_head(o.tol, o.maxit); })) ||
+            if ((rc = capture_fn(&gh, [&]() { return enqueue_step_head(o.tol, o.maxit, o.predictor); })) ||
                 (rc = capture_fn(&gt, [&]() { return enqueue_step_tail(); }))) {
                 cleanup();
                 return rc;
@@ -677,7 +683,7 @@ struct Solver final : SolverBase {
             while (t < o.t_end * (1 - 1e-12) && (o.max_steps <= 0 || steps < o.max_steps)) {
                 dt = std::min(dt, o.t_end - t);
                 if (gh) HIPTRY(hipGraphLaunch(gh, s));
-                else if ((rc = enqueue_step_head(o.tol, o.maxit))) return rc;
+                else if ((rc = enqueue_step_head(o.tol, o.maxit, o.predictor))) return rc;
                 for (int q = 0; q < std::min(prev_iters, o.maxit); ++q)
                     if ((rc = run_iter())) return rc;
                 if ((rc = fetch_state())) return rc;

@@ -175,6 +175,7 @@ void th_transient_default_opts(th_transient_opts* o) {
     o->tol = 1e-9;
     o->maxit = 50000;
     o->max_steps = 0;
+    o->predictor = 0;
 }
 
 int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc, const double* zc_coarse,
@@ -205,7 +206,7 @@ int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* r
                      double* T_final, int* all_converged, double* solve_ms) {
     TH_S(s);
     if (!o || !nsteps) return fail(TH_ERR_ARG, "null argument");
-    if (!(o->dt0 > 0) || !(o->growth > 0) || !(o->dt_max > 0) || o->maxit < 0)
+    if (!(o->dt0 > 0) || !(o->growth > 0) || !(o->dt_max > 0) || o->maxit < 0 || o->predictor < 0 || o->predictor > 1)
         return fail(TH_ERR_ARG, "transient options out of range");
     int rc = s->impl->run_transient(*o, recs, rec_cap, nsteps, all_converged, solve_ms);
     if (rc) return rc;

@@ -64,7 +64,7 @@ class SolveResult(C.Structure):
 
 class TransientOpts(C.Structure):
     _fields_ = [("dt0", C.c_double), ("growth", C.c_double), ("dt_max", C.c_double), ("t_end", C.c_double),
-                ("tol", C.c_double), ("maxit", C.c_int), ("max_steps", C.c_int)]
+                ("tol", C.c_double), ("maxit", C.c_int), ("max_steps", C.c_int), ("predictor", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -472,9 +472,10 @@ def steady_solve(model, tol=1e-10, maxit=50000, **kw):
 
 
 def run_transient(solver, dt0=1e-5, growth=1.1, dt_max=0.05, t_end=1.0, tol=1e-9, maxit=50000, max_steps=0,
-                  rec_cap=100000):
-    """run_transient (transient3d.h:48-90) on a Solver; energy terms on the device."""
-    o = TransientOpts(dt0, growth, dt_max, t_end, tol, maxit, max_steps)
+                  rec_cap=100000, predictor=False):
+    """run_transient (transient3d.h:48-90) on a Solver; energy terms on the device.
+    predictor: start each step from the linear extrapolation of the last two."""
+    o = TransientOpts(dt0, growth, dt_max, t_end, tol, maxit, max_steps, int(predictor))
     recs = (StepRecord * rec_cap)()
     n, conv = C.c_int(), C.c_int()
     ms = C.c_double()

@@ -198,6 +198,8 @@ typedef struct {
     double tol;
     int maxit;
     int max_steps;                     /* > 0: also stop after this many steps */
+    int predictor;                     /* start of each step: 0 = T_old (the reference's warm
+                                          start); 1 = linear extrapolation of the last two steps */
 } th_transient_opts;
 TH_API void th_transient_default_opts(th_transient_opts* o);
 

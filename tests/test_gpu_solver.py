@@ -193,6 +193,24 @@ def test_transient_step_rule_vs_oracle():
     assert np.allclose(g["q_out"], ref["q_out"], rtol=1e-7, atol=1e-9 * m.total_power())
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_transient_predictor(graphs):
+    """Steps started from the extrapolation of the last two reach the same
+    answers (to the solve tolerance) in fewer iterations."""
+    m = hotspot(48, at=(5e-3, 5e-3))
+    a = m.arrays()
+    kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    ref = O.run_transient(a, flexible=True, **kw)
+    plain = T.run_transient(T.Solver(m, graphs=graphs), **kw)
+    pred = T.run_transient(T.Solver(m, graphs=graphs), predictor=True, **kw)
+    assert pred["converged"] and np.array_equal(pred["t"], ref["t"])
+    assert np.abs(pred["T"] - ref["T"]).max() < 1e-7 * ref["T"].max()
+    assert np.allclose(pred["T_max"], ref["T_max"], rtol=1e-7)
+    P = m.total_power()
+    assert (np.abs(pred["q_out"] + pred["q_stored"] - P) / P).max() < 1e-6
+    assert pred["iters"].sum() < 0.95 * plain["iters"].sum(), (pred["iters"].sum(), plain["iters"].sum())
+
+
 def test_transient_800x800x10_100_steps():
     """The 800x800x10 transient from ambient, 100 backward-Euler steps:
     per-step energy balance, mixed vs FP64 steps, and the final field
