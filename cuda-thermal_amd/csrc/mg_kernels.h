@@ -121,6 +121,8 @@ __device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__
 // recurrence from LDS (the same arithmetic in the same order as
 // sweep_column) and every lane writes its plane back.
 constexpr int kPPThreads = 256;
+constexpr long kPPCols = 8192;  // levels with at most this many columns per colour sweep plane-parallel
+__host__ __device__ constexpr int pp_lanes(int nz) { return nz <= 4 ? 4 : nz <= 8 ? 8 : 16; }
 template <class T, int LP, int MODE>
 __device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const Ops<T>& o, const T* __restrict__ finv,
                                               const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,

@@ -354,17 +354,15 @@ struct Solver final : SolverBase {
         int mode;
     };
 
-    // Levels with at most this many columns per colour run the plane-parallel
+    // Levels with at most kPPCols columns per colour run the plane-parallel
     // sweep (k_sweep_pp); larger ones the column walk (k_sweep).
-    static constexpr long kPPCols = 8192;
-
     template <int MODE>
     int launch_sweep(size_t k, int colour, Cons cs = {nullptr, 0, CONS_NONE}) {
         const DevLevel<T>& d = L[k];
         const DevLevel<T>& c = L[std::min(k + 1, L.size() - 1)];
         const long cols = (long)d.g.hx * d.g.ny;
         if (cols <= kPPCols && d.g.nz <= 16) {
-            const unsigned nb = (unsigned)((cols * (d.g.nz <= 4 ? 4 : d.g.nz <= 8 ? 8 : 16) + kPPThreads - 1) / kPPThreads);
+            const unsigned nb = (unsigned)((cols * pp_lanes(d.g.nz) + kPPThreads - 1) / kPPThreads);
 #define TH_PP(LP)                                                                                                   \
     k_sweep_pp<T, LP, MODE><<<nb, kPPThreads, 0, s>>>(d.g, d.ops(), d.finv, d.b, d.x, colour, c.g, c.x, st, cs.part, \
                                                       cs.nblk, cs.mode)
