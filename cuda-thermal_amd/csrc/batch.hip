@@ -98,11 +98,12 @@ __global__ __launch_bounds__(512) void kb_restrict(const Inst* __restrict__ I, c
 template <int NZ>
 __global__ __launch_bounds__(kTailLdsThreads) void kb_tail(const Inst* __restrict__ I,
                                                            const Tail<float>* __restrict__ tails, int cons) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const Inst& in = I[blockIdx.x];
     const double* cp = cons == CONS_START ? in.part_res : in.part_upd;
     const int cn = cons == CONS_START ? in.nblk_stencil : in.nblk_vec;
-    tail_lds_body<NZ>(tails[blockIdx.x], in.st, cp, cn, in.tail_k0 == 0 ? cons : (int)CONS_NONE, sm);
+    tail_lds_body<float, NZ>(tails[blockIdx.x], in.st, cp, cn, in.tail_k0 == 0 ? cons : (int)CONS_NONE,
+                             reinterpret_cast<float*>(smem));
 }
 
 __global__ __launch_bounds__(1024) void kb_residual(const Inst* __restrict__ I, const Entry* __restrict__ tab,

@@ -24,7 +24,7 @@ enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 constexpr int kTailThreads = 512;
 constexpr int kMaxTail = 8;
 constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
-constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 4 KB static)
+constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 4 or 8 KB static)
 constexpr int kTailLdsThreads = 1024;           // plane-parallel sweeps: 64 / LP columns per wave
 
 // Coarse parents of the 4 lateral neighbours of column (i, j) (SW_PROLONG).
@@ -384,35 +384,38 @@ __global__ __launch_bounds__((tail_threads<T, NZ>())) void k_tail(Tail<T> t, Sta
 }
 
 // LDS bytes of an LDS-resident tail over levels g[0..n-1]: per cell gE, gN,
-// gU, 1/pivot, x, b (the fast restriction reads no gB and no capacity).
+// gU, 1/pivot, x, b (the lateral restriction reads no gB and no capacity).
+template <class T>
 inline size_t tail_lds_bytes(const Geo* g, int n) {
     size_t b = 0;
-    for (int k = 0; k < n; ++k) b += 6 * sizeof(float) * (size_t)g[k].cells();
+    for (int k = 0; k < n; ++k) b += 6 * sizeof(T) * (size_t)g[k].cells();
     return b;
 }
 
-// The same tail with every level copied into LDS first (FP32, lateral
+// The same tail with every level copied into LDS first (lateral
 // restriction), every sweep plane-parallel: LP lanes per column gather the
 // planes' right-hand sides at once, the column's first lane runs the Thomas
 // recurrence (sweep_column's arithmetic), every lane writes its plane back.
 // Level arrays are addressed from the dynamic LDS base, so every access is
 // an LDS access. Only the top level's x goes back to global memory, for the
 // prolongation into the level above.
+template <class T>
 struct LdsLevel {
     Geo g;
-    float *gE, *gN, *gU, *fi, *x, *b;
+    T *gE, *gN, *gU, *fi, *x, *b;
 };
-__device__ __forceinline__ LdsLevel lds_level(const Tail<float>& t, float* sm, int k) {
+template <class T>
+__device__ __forceinline__ LdsLevel<T> lds_level(const Tail<T>& t, T* sm, int k) {
     size_t off = 0;
     for (int m = 0; m < k; ++m) off += 6 * (size_t)t.L[m].g.cells();
     const Geo g = t.L[k].g;
     const long cells = g.cells();
-    float* p = sm + off;
-    return LdsLevel{g, p, p + cells, p + 2 * cells, p + 3 * cells, p + 4 * cells, p + 5 * cells};
+    T* p = sm + off;
+    return LdsLevel<T>{g, p, p + cells, p + 2 * cells, p + 3 * cells, p + 4 * cells, p + 5 * cells};
 }
 
-template <int LP, int MODE>
-__device__ __forceinline__ void tail_sweep_pp(const LdsLevel& F, int colour, const LdsLevel& C, float* scr_s) {
+template <class T, int LP, int MODE>
+__device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s) {
     constexpr int CPP = kTailLdsThreads / LP;  // columns per pass
     const int tid = threadIdx.x, l = tid % LP;
     const Geo& g = F.g;
@@ -427,23 +430,41 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel& F, int colour, con
         if (colok) c = make_col(g, i, j);
         if (ok) {
             const Parents pa = MODE == SW_PROLONG ? parents(C.g, c, i, j) : Parents{0, 0, 0, 0};
-            scr_s[tid] = column_rhs<float, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
+            scr_s[tid] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
         }
         __syncthreads();
         if (colok && l == 0) {
-            float sv[LP], iv[LP], uv[LP];
+            if constexpr (sizeof(T) == 4) {
+                T sv[LP], iv[LP], uv[LP];
 #pragma unroll
-            for (int k = 0; k < LP; ++k) {
-                sv[k] = scr_s[tid + k];
-                if (k < g.nz) {
+                for (int k = 0; k < LP; ++k) {
+                    sv[k] = scr_s[tid + k];
+                    if (k < g.nz) {
+                        const long id = (long)k * g.plane + c.c0;
+                        iv[k] = F.fi[id];
+                        uv[k] = F.gU[id];
+                    }
+                }
+                column_thomas<T, LP>(g.nz, sv, iv, uv);
+#pragma unroll
+                for (int k = 0; k < LP; ++k) scr_s[tid + k] = sv[k];
+            } else {
+                // Three FP64 columns of 16 planes exceed a 1024-thread
+                // block's 128 registers: the recurrence (column_thomas's
+                // arithmetic, in its order) runs in place in LDS instead.
+                const int nz = g.nz;
+                T prev = 0;
+                for (int k = 0; k < nz; ++k) {
                     const long id = (long)k * g.plane + c.c0;
-                    iv[k] = F.fi[id];
-                    uv[k] = F.gU[id];
+                    const T lower = (k > 0) ? -F.gU[id - g.plane] : T(0);
+                    prev = (scr_s[tid + k] - lower * prev) * F.fi[id];
+                    scr_s[tid + k] = prev;
+                }
+                for (int k = nz - 2; k >= 0; --k) {
+                    const long id = (long)k * g.plane + c.c0;
+                    scr_s[tid + k] = scr_s[tid + k] - ((-F.gU[id]) * F.fi[id]) * scr_s[tid + k + 1];
                 }
             }
-            column_thomas<float, LP>(g.nz, sv, iv, uv);
-#pragma unroll
-            for (int k = 0; k < LP; ++k) scr_s[tid + k] = sv[k];
         }
         __syncthreads();
         if (ok) F.x[(long)l * g.plane + c.c0] = scr_s[tid];
@@ -451,16 +472,16 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel& F, int colour, con
     __syncthreads();
 }
 
-template <int NZ>
-__device__ __forceinline__ void tail_lds_body(const Tail<float>& t, State* st, const double* __restrict__ cons_part,
-                                              int cons_nblk, int cons_mode, float* sm) {
-    constexpr int LP = NZ <= 4 ? 4 : NZ <= 8 ? 8 : 16;
-    __shared__ float scr_s[kTailLdsThreads];
+template <class T, int NZ>
+__device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const double* __restrict__ cons_part,
+                                              int cons_nblk, int cons_mode, T* sm) {
+    constexpr int LP = pp_lanes(NZ);
+    __shared__ T scr_s[kTailLdsThreads];
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
     for (int k = 0; k < t.n; ++k) {
-        const TailLevel<float>& G = t.L[k];
-        const LdsLevel D = lds_level(t, sm, k);
+        const TailLevel<T>& G = t.L[k];
+        const LdsLevel<T> D = lds_level<T>(t, sm, k);
         const long cells = G.g.cells();
         for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
             D.gE[q] = G.o.gE[q];
@@ -473,34 +494,34 @@ __device__ __forceinline__ void tail_lds_body(const Tail<float>& t, State* st, c
     __syncthreads();
     const int n = t.n;
     for (int k = 0; k + 1 < n; ++k) {
-        const LdsLevel F = lds_level(t, sm, k), C = lds_level(t, sm, k + 1);
-        tail_sweep_pp<LP, SW_ZERO>(F, 0, C, scr_s);
-        tail_sweep_pp<LP, SW_PLAIN>(F, 1, C, scr_s);
+        const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
+        tail_sweep_pp<T, LP, SW_ZERO>(F, 0, C, scr_s);
+        tail_sweep_pp<T, LP, SW_PLAIN>(F, 1, C, scr_s);
         const int cells = C.g.nx * C.g.ny * C.g.nz;
         for (int q = threadIdx.x; q < cells; q += kTailLdsThreads) {
             const int I = q % C.g.nx, rest = q / C.g.nx;
-            restrict_cell_fast<float>(F.g, F.gE, F.gN, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny);
+            restrict_cell_fast<T>(F.g, F.gE, F.gN, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny);
         }
         __syncthreads();
     }
     {
-        const LdsLevel N = lds_level(t, sm, n - 1);
-        tail_sweep_pp<LP, SW_ZERO>(N, 0, N, scr_s);
+        const LdsLevel<T> N = lds_level<T>(t, sm, n - 1);
+        tail_sweep_pp<T, LP, SW_ZERO>(N, 0, N, scr_s);
     }
     for (int k = n - 2; k >= 0; --k) {
-        const LdsLevel F = lds_level(t, sm, k), C = lds_level(t, sm, k + 1);
-        tail_sweep_pp<LP, SW_PROLONG>(F, 1, C, scr_s);
-        tail_sweep_pp<LP, SW_PLAIN>(F, 0, C, scr_s);
+        const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
+        tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s);
+        tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s);
     }
-    const LdsLevel T0 = lds_level(t, sm, 0);
+    const LdsLevel<T> T0 = lds_level<T>(t, sm, 0);
     const long cells0 = t.L[0].g.cells();
     for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = T0.x[q];
 }
-template <int NZ>
-__global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<float> t, State* st, const double* __restrict__ cons_part,
+template <class T, int NZ>
+__global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<T> t, State* st, const double* __restrict__ cons_part,
                                                    int cons_nblk, int cons_mode) {
-    extern __shared__ float sm[];
-    tail_lds_body<NZ>(t, st, cons_part, cons_nblk, cons_mode, sm);
+    extern __shared__ __align__(16) unsigned char smem[];
+    tail_lds_body<T, NZ>(t, st, cons_part, cons_nblk, cons_mode, reinterpret_cast<T*>(smem));
 }
 
 }  // namespace th

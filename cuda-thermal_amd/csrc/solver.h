@@ -216,20 +216,20 @@ struct Solver final : SolverBase {
         // Otherwise: levels with at most kTailCols columns (k_tail, L2).
         fast = opt.restriction == TH_RESTRICT_LATERAL;
         tail_k0 = L.size() - 1;
-        lds_tail = fast && sizeof(T) == 4 && g0.nz <= 16;
+        lds_tail = fast && g0.nz <= 16;
         if (lds_tail) {
             auto bytes_from = [&](size_t k0) {
                 std::vector<Geo> gs;
                 for (size_t k = k0; k < L.size(); ++k) gs.push_back(L[k].g);
-                return tail_lds_bytes(gs.data(), (int)gs.size());
+                return tail_lds_bytes<T>(gs.data(), (int)gs.size());
             };
             while (tail_k0 > 0 && bytes_from(tail_k0 - 1) <= kTailLdsBytes && L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
                 --tail_k0;
             lds_bytes = bytes_from(tail_k0);
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<8>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<12>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<16>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<T, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<T, 12>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
+            HIPTRY(hipFuncSetAttribute((const void*)k_tail_lds<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
         } else {
             while (tail_k0 > 0 && (long)L[tail_k0 - 1].g.nx * L[tail_k0 - 1].g.ny <= kTailCols &&
                    L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
@@ -399,8 +399,7 @@ struct Solver final : SolverBase {
         }
         const int nz = g0.nz;
         if (lds_tail) {
-            const Tail<float>& tf = *reinterpret_cast<const Tail<float>*>(&t);
-#define TH_TAIL(NZ) k_tail_lds<NZ><<<1, kTailLdsThreads, lds_bytes, s>>>(tf, st, cs.part, cs.nblk, cs.mode)
+#define TH_TAIL(NZ) k_tail_lds<T, NZ><<<1, kTailLdsThreads, lds_bytes, s>>>(t, st, cs.part, cs.nblk, cs.mode)
             if (nz <= 4) TH_TAIL(4);
             else if (nz <= 8) TH_TAIL(8);
             else if (nz <= 12) TH_TAIL(12);
