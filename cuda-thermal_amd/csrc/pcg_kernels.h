@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kVecThreads) void k_energy(Geo g, Ops<T> o, const d
         const double xv = x[k];
         const long q = k % g.plane;
         int i;
-        const bool cell = slot_column(g, (int)(q % (2 * g.hx)), (int)(q / (2 * g.hx)), i);
+        const bool cell = q < g.slots() && slot_column(g, (int)(q % (2 * g.hx)), (int)(q / (2 * g.hx)), i);
         v[0] += (double)o.cap[k] * xv;
         if (k >= top) v[1] += (double)o.gU[k] * xv;
         if (k < g.plane) v[1] += (double)o.gB[k] * xv;
