@@ -713,15 +713,17 @@ struct Solver final : SolverBase {
                      int* used, double* rp, double* rl) override {
         int rc;
         double r_plain = 1.0, r_lift = 0.0;  // x = 0: P - A x = P
-        if (!x_zero && (rc = true_relres(&r_plain))) return rc;
-        HIPTRY(hipMemcpyAsync(Ap, x, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, x);
+        if (!x_zero && (rc = relres_of(x, &r_plain))) return rc;
+        // The lifted guess is built and judged in Ap (free between solves):
+        // x is only overwritten when the guess is kept.
+        double* v = Ap;
+        k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, v);
         HIPTRY(hipGetLastError());
         const Ops<T> o = L[0].ops();
         for (int k = 0; k < passes; ++k)
             for (int colour = 0; colour < 2; ++colour) {
                 const dim3 grid = col_grid(g0.hx, g0.ny);
-#define TH_RELAX(NZ) k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, x, colour)
+#define TH_RELAX(NZ) k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, v, colour)
                 if (g0.nz <= 4) TH_RELAX(4);
                 else if (g0.nz <= 8) TH_RELAX(8);
                 else if (g0.nz <= 12) TH_RELAX(12);
@@ -733,10 +735,12 @@ struct Solver final : SolverBase {
 #undef TH_RELAX
                 HIPTRY(hipGetLastError());
             }
-        if ((rc = true_relres(&r_lift))) return rc;
+        if ((rc = relres_of(v, &r_lift))) return rc;
         const bool keep = r_lift < r_plain;
-        if (!keep) HIPTRY(hipMemcpyAsync(x, Ap, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        else x_zero = false;
+        if (keep) {
+            HIPTRY(hipMemcpyAsync(x, v, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+            x_zero = false;
+        }
         HIPTRY(hipStreamSynchronize(s));
         if (used) *used = keep;
         if (rp) *rp = r_plain;
@@ -745,9 +749,11 @@ struct Solver final : SolverBase {
     }
     int nz() const override { return g0.nz; }
 
-    int true_relres(double* out) override {
+    int true_relres(double* out) override { return relres_of(x, out); }
+    // ||P - A v|| / ||P|| of the steady operator (flux form)
+    int relres_of(const double* v, double* out) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, x, nullptr, nullptr, nullptr,
+        k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, v, nullptr, nullptr, nullptr,
                                                                   nullptr, nullptr, st, part);
         HIPTRY(hipGetLastError());
         int rc;

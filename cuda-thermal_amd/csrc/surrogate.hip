@@ -290,6 +290,15 @@ __global__ void k_gather(const bf* __restrict__ ds, const int* __restrict__ idx,
     X0[k] = c < kIn ? ds[((long)idx[img] * hw + p) * kInStore + c] : (bf)0;
 }
 
+// Network input from FP32 features [B][H][W][3]: 16 BF16 channels, 3 used.
+__global__ void k_pack_input(const float* __restrict__ in, bf* __restrict__ X0, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = (int)(k & 15);
+    const long pix = k >> 4;
+    X0[k] = c < kIn ? f2bf(in[pix * kIn + c]) : (bf)0;
+}
+
 // MSE over the 10 used outputs: gradient 2 (out - tgt) / count in BF16 and
 // per-block partial sums of the squared error. idx == nullptr: targets are
 // batch-contiguous.
@@ -404,6 +413,9 @@ struct Net {
     double* dloss = nullptr;  // per-step losses (device)
     int dloss_cap = 0;
     float* tgt_b = nullptr;  // batch-contiguous targets (gradient checks)
+    float* in_stage = nullptr;  // FP32 features of a batch (device)
+    float* in_pin = nullptr;    // and their pinned host staging
+    size_t in_pin_n = 0;
     int* d_idx = nullptr;
     // dataset
     bf* ds_in = nullptr;
@@ -418,6 +430,7 @@ struct Net {
         if (s) (void)hipStreamSynchronize(s);
         free_acts();
         for (void* p : own) (void)hipFree(p);
+        if (in_pin) (void)hipHostFree(in_pin);
         if (s) (void)hipStreamDestroy(s);
     }
     void free_acts() {
@@ -515,7 +528,8 @@ struct Net {
         for (const A& a : acts)
             if ((rc = dalloc(a.p, a.n, act_mem))) return rc;
         if ((rc = dalloc(&out, n0 * 16, act_mem)) || (rc = dalloc(&tgt_b, n0 * kOut, act_mem)) ||
-            (rc = dalloc(&lpart, grid1(n0 * 16), act_mem)) || (rc = dalloc(&d_idx, B, act_mem)))
+            (rc = dalloc(&lpart, grid1(n0 * 16), act_mem)) || (rc = dalloc(&d_idx, B, act_mem)) ||
+            (rc = dalloc(&in_stage, n0 * kIn, act_mem)))
             return rc;
         Bcap = B;
         return TH_OK;
@@ -693,11 +707,22 @@ struct Net {
     }
 
     // host features [B][H][W][3] -> x0 (BF16, 16 channels)
+    // host features [B][H][W][3] -> pinned staging -> device, packed to x0
+    // on the device; returns with the staging buffer free again.
     int upload_input(const float* in, int B) {
-        std::vector<bf> h((size_t)npix(0, B) * 16, 0);
-        for (long p = 0; p < npix(0, B); ++p)
-            for (int c = 0; c < kIn; ++c) h[p * 16 + c] = host_f2bf(in[p * kIn + c]);
-        SURTRY(hipMemcpyAsync(x0, h.data(), h.size() * sizeof(bf), hipMemcpyHostToDevice, s));
+        const size_t n = (size_t)npix(0, B) * kIn;
+        if (n > in_pin_n) {
+            SURTRY(hipStreamSynchronize(s));
+            if (in_pin) (void)hipHostFree(in_pin);
+            in_pin = nullptr;
+            in_pin_n = 0;
+            SURTRY(hipHostMalloc((void**)&in_pin, n * sizeof(float), hipHostMallocDefault));
+            in_pin_n = n;
+        }
+        std::memcpy(in_pin, in, n * sizeof(float));
+        SURTRY(hipMemcpyAsync(in_stage, in_pin, n * sizeof(float), hipMemcpyHostToDevice, s));
+        k_pack_input<<<grid1(npix(0, B) * 16), 256, 0, s>>>(in_stage, x0, npix(0, B) * 16);
+        SURTRY(hipGetLastError());
         SURTRY(hipStreamSynchronize(s));
         return TH_OK;
     }
