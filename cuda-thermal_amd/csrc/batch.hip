@@ -39,7 +39,8 @@ struct Inst {
     Geo g0;
     long n0;
     const double* P;
-    double *x, *b, *r, *p, *Ap;
+    double *x, *b, *r, *Ap;
+    float* p;  // search direction (FP32, as the single-instance mixed solver)
     double *part_res, *part_apply, *part_upd, *part_dot;
     int nblk_stencil, nblk_vec;
     int nlev, tail_k0;
@@ -118,13 +119,13 @@ template <int NZ>
 __global__ __launch_bounds__(1024) void kb_apply(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    apply_body<float, NZ>(B, in.g0, in.L[0].o, in.p, in.Ap, in.st, in.part_apply);
+    apply_body<float, NZ, float>(B, in.g0, in.L[0].o, in.p, in.Ap, in.st, in.part_apply);
 }
 
 __global__ __launch_bounds__(1024) void kb_update(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    update_body<float, UP_MG_FLEX>(B, in.n0, in.p, in.Ap, in.x, in.r, in.L[0].b, in.L[0].x, nullptr, nullptr, in.st,
+    update_body<float, UP_MG_FLEX>(B, in.n0, nullptr, in.Ap, in.x, in.r, in.L[0].b, in.L[0].x, nullptr, nullptr, in.st,
                                    in.part_upd, in.part_apply, in.nblk_stencil);
 }
 
@@ -139,7 +140,8 @@ __global__ __launch_bounds__(1024) void kb_pupdate(const Inst* __restrict__ I, c
                                                    int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    pupdate_body<float, FIRST>(B, in.n0, in.L[0].x, in.p, in.st, in.part_dot, in.nblk_vec);
+    pupdate_body<float, FIRST, float>(B, in.n0, in.L[0].x, in.p, FIRST ? nullptr : in.x, in.st, in.part_dot,
+                                      in.nblk_vec);
 }
 
 // Steady right-hand side and zero start for every instance: b = P, x = 0.
@@ -153,19 +155,25 @@ __global__ __launch_bounds__(1024) void kb_reset(const Inst* __restrict__ I, con
     }
 }
 
-// x = 0 where b = 0 was detected (as k_finish).
+// x = 0 where b = 0 was detected, else the last update's pending
+// x += alpha p (as k_finish).
 __global__ __launch_bounds__(1024) void kb_finish(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    if (!in.st->zero_x) return;
     const long stride = (long)B.gx * blockDim.x;
-    for (long k = (long)B.x * blockDim.x + threadIdx.x; k < in.n0; k += stride) in.x[k] = 0.0;
+    if (in.st->zero_x) {
+        for (long k = (long)B.x * blockDim.x + threadIdx.x; k < in.n0; k += stride) in.x[k] = 0.0;
+    } else if (in.st->xlag) {
+        const double alpha = in.st->alpha;
+        for (long k = (long)B.x * blockDim.x + threadIdx.x; k < in.n0; k += stride) in.x[k] += alpha * (double)in.p[k];
+    }
 }
 
 __global__ void kb_begin(const Inst* __restrict__ I, int n, double tol, int maxit) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     State* st = I[i].st;
+    st->xlag = 0;
     st->tol = tol;
     st->maxit = maxit;
     st->iters = 0;
@@ -275,7 +283,7 @@ struct Batch {
             d.x = x.x;
             d.b = x.b;
             d.r = x.r;
-            d.p = x.p;
+            d.p = x.pm;
             d.Ap = x.Ap;
             d.part_res = x.part_res;
             d.part_apply = x.part_apply;

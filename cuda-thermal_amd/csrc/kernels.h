@@ -78,6 +78,7 @@ struct State {
     double chk_rr, chk_bb;         // true-residual check
     double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
     int maxit, iters, done, converged, zero_x, flexible;
+    int xlag;  // multigrid path: x += alpha p of the last update is still to be applied
     // Transient stepping on the device (transient3d.h:57-84): the time, the
     // step size of the current operator and of the next step, the stepping
     // rule, the heat content after the last step and that step's record.

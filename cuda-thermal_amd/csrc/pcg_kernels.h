@@ -75,8 +75,8 @@ __global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double
 }
 
 // Ap = (A + C/dt) p; partial p.Ap.
-template <class T, int NZ>
-__device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops<T>& o, const double* __restrict__ p,
+template <class T, int NZ, class PT = double>
+__device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops<T>& o, const PT* __restrict__ p,
                                            double* __restrict__ Ap, const State* __restrict__ st,
                                            double* __restrict__ part) {
     const int nz = g.nz;
@@ -92,16 +92,17 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
     int i;
     if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
         const Col c = make_col(g, i, j);
-        double pb = 0.0, pc = p[c.c0], gdn = (double)o.gB[c.c0];
+        double pb = 0.0, pc = (double)p[c.c0], gdn = (double)o.gB[c.c0];
 #pragma unroll
         for (int l = 0; l < NZ; ++l) {
             if (l < nz) {
                 const long base = (long)l * g.plane, id = base + c.c0;
-                const double pu = (l < nz - 1) ? p[id + g.plane] : 0.0;
+                const double pu = (l < nz - 1) ? (double)p[id + g.plane] : 0.0;
                 const double gu = (double)gU[id];
                 const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id];
                 const double Gs = (double)gN[base + c.ps], Gn = (double)gN[id];
-                const double xw = p[base + c.pw], xe = p[base + c.pe], xs = p[base + c.ps], xn = p[base + c.pn];
+                const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
+                             xn = (double)p[base + c.pn];
                 double d = 0.0, s = 0.0;
                 if (c.hw) d += Gw;
                 if (c.he) d += Ge;
@@ -127,18 +128,20 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
     }
     block_partials<1>(v, part, B.flat());
 }
-template <class T, int NZ>
-__global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const double* __restrict__ p,
+template <class T, int NZ, class PT = double>
+__global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const PT* __restrict__ p,
                                                 double* __restrict__ Ap, const State* __restrict__ st,
                                                 double* __restrict__ part) {
-    apply_body<T, NZ>(hw_blk(), g, o, p, Ap, st, part);
+    apply_body<T, NZ, PT>(hw_blk(), g, o, p, Ap, st, part);
 }
 
-// x += alpha p, r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or
-// z = r / diag (Jacobi); partials r.r and r.z_old (flexible beta: z_old is the
+// r -= alpha Ap, and b0 = (T) r for the V-cycle (MG) or z = r / diag and
+// x += alpha p (Jacobi); partials r.r and r.z_old (flexible beta: z_old is the
 // previous V-cycle output, still in x0) or r.z (Jacobi).
 // Multigrid path: alpha = rz / p.Ap is formed in every block from the apply
-// kernel's partials (apart), so no reduction launch sits between the two.
+// kernel's partials (apart), so no reduction launch sits between the two;
+// x += alpha p is left to the p update, which reads p anyway (xlag marks it
+// pending until then; k_finish applies it when the solve stops first).
 template <class T, int MODE>
 __device__ __forceinline__ void update_body(const Blk& B, long n, const double* __restrict__ p,
                                             const double* __restrict__ Ap, double* __restrict__ x,
@@ -155,6 +158,7 @@ __device__ __forceinline__ void update_body(const Blk& B, long n, const double* 
         if (B.x == 0 && threadIdx.x == 0) {
             st->pAp = pAp[0];
             st->alpha = alpha;
+            if (MODE != UP_JACOBI) st->xlag = 1;
         }
     } else {
         alpha = st->alpha;
@@ -162,7 +166,7 @@ __device__ __forceinline__ void update_body(const Blk& B, long n, const double* 
     double v[2] = {0.0, 0.0};
     const long stride = (long)B.gx * blockDim.x;
     for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        x[k] += alpha * p[k];
+        if (MODE == UP_JACOBI) x[k] += alpha * p[k];
         const double ri = r[k] - alpha * Ap[k];
         r[k] = ri;
         v[0] += ri * ri;
@@ -207,9 +211,13 @@ __global__ __launch_bounds__(kVecThreads) void k_dot(long n, const double* __res
 // the dot kernel's partials (dpart) and beta from it - Fletcher-Reeves
 // rz_new / rz_old or flexible (rz_new - r.z_old) / rz_old - with rz kept by
 // iteration parity, so blocks never read a value another block rewrites.
-template <class Z, int FIRST>
-__device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __restrict__ z, double* __restrict__ p,
-                                             State* st, const double* __restrict__ dpart, int dnblk) {
+// x (multigrid, not FIRST): the update's x += alpha p is applied here, from
+// the p it reads anyway. p is stored in PT (FP32 in the mixed solver: x and
+// r both move by the rounded p, so r stays the residual of x).
+template <class Z, int FIRST, class PT = double>
+__device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __restrict__ z, PT* __restrict__ p,
+                                             double* __restrict__ x, State* st, const double* __restrict__ dpart,
+                                             int dnblk) {
     if (st->done) return;
     double beta = 0.0;
     if (dpart) {
@@ -229,13 +237,24 @@ __device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __re
         beta = st->beta;
     }
     const long stride = (long)B.gx * blockDim.x;
+    if (!FIRST && x) {
+        const double alpha = st->alpha;
+        for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+            const double po = (double)p[k];
+            x[k] += alpha * po;
+            p[k] = (PT)((double)z[k] + beta * po);
+        }
+        if (B.x == 0 && threadIdx.x == 0) st->xlag = 0;
+        return;
+    }
     for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride)
-        p[k] = FIRST ? (double)z[k] : (double)z[k] + beta * p[k];
+        p[k] = FIRST ? (PT)z[k] : (PT)((double)z[k] + beta * (double)p[k]);
 }
-template <class Z, int FIRST>
-__global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __restrict__ z, double* __restrict__ p,
-                                                          State* st, const double* __restrict__ dpart, int dnblk) {
-    pupdate_body<Z, FIRST>(hw_blk(), n, z, p, st, dpart, dnblk);
+template <class Z, int FIRST, class PT = double>
+__global__ __launch_bounds__(kVecThreads) void k_pupdate(long n, const Z* __restrict__ z, PT* __restrict__ p,
+                                                          double* __restrict__ x, State* st,
+                                                          const double* __restrict__ dpart, int dnblk) {
+    pupdate_body<Z, FIRST, PT>(hw_blk(), n, z, p, x, st, dpart, dnblk);
 }
 
 // Adds the partials of the previous kernel and runs its epilogue.
@@ -300,6 +319,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
 
 // Start of a solve: tolerance, iteration cap, cleared flags.
 static __global__ void k_begin(State* st, double tol, int maxit) {
+    st->xlag = 0;
     st->tol = tol;
     st->maxit = maxit;
     st->iters = 0;
@@ -310,6 +330,7 @@ static __global__ void k_begin(State* st, double tol, int maxit) {
 }
 // Reopens a finished solve for a restart from the current x (iterations kept).
 static __global__ void k_reopen(State* st) {
+    st->xlag = 0;
     st->done = 0;
     st->converged = 0;
 }
@@ -337,6 +358,7 @@ static __global__ void k_step_init(State* st, double dt0, double growth, double 
 // operator's 1/dt and asks for new factors (set_dt only if dt changed, :61).
 // Then tolerance and flags of the step's solve, as k_begin.
 static __global__ void k_step_begin(State* st, double tol, int maxit) {
+    st->xlag = 0;
     const double dt = fmin(st->dt_next, st->t_end - st->t);
     st->dt_prev = st->dt;
     st->refactor = dt != st->dt;
@@ -385,10 +407,15 @@ static __global__ __launch_bounds__(kVecThreads) void k_predict(long n, double* 
 }
 
 // x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).
-static __global__ void k_finish(long n, double* __restrict__ x, const State* __restrict__ st) {
-    if (!st->zero_x) return;
+template <class PT>
+__global__ void k_finish(long n, double* __restrict__ x, const PT* __restrict__ p, const State* __restrict__ st) {
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = 0.0;
+    if (st->zero_x) {
+        for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = 0.0;
+    } else if (st->xlag) {  // the last update's x += alpha p
+        const double alpha = st->alpha;
+        for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] += alpha * (double)p[k];
+    }
 }
 
 // Jacobi: 1 / diag of (A + C/dt) (pcg3d_cpu.cpp:41-48).

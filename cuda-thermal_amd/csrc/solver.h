@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -104,6 +105,7 @@ struct Solver final : SolverBase {
     double inv_dt = 0;
     double *P = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
     double *z = nullptr, *invd = nullptr, *xnat = nullptr;
+    T* pm = nullptr;  // multigrid search direction, in the V-cycle's precision (p itself in FP64)
     double* part = nullptr;
     State* st = nullptr;
     State* h_st = nullptr;  // pinned mirror
@@ -163,6 +165,10 @@ struct Solver final : SolverBase {
         if (!mg())
             for (double** v : {&z, &invd})
                 if ((rc = dalloc(v, n0))) return rc;
+        if (mg()) {
+            if constexpr (std::is_same<T, double>::value) pm = p;
+            else if ((rc = dalloc(&pm, n0))) return rc;
+        }
         if ((rc = dalloc(&st, 1))) return rc;
         HIPTRY(hipHostMalloc((void**)&h_st, sizeof(State), hipHostMallocDefault));
         std::memset(h_st, 0, sizeof(State));
@@ -452,11 +458,12 @@ struct Solver final : SolverBase {
         return launch_sweep<SW_PLAIN>(k, 0);
     }
 
-    // Ap = (A + C/dt) p with partials p.Ap in part_apply.
-    int launch_apply_kernel() {
+    // Ap = (A + C/dt) v with partials v.Ap in part_apply.
+    template <class PT>
+    int launch_apply_kernel(const PT* v) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const Ops<T> o = L[0].ops();
-#define TH_APPLY(NZ) k_apply<T, NZ><<<grid, stencil_block(), 0, s>>>(g0, o, p, Ap, st, part_apply)
+#define TH_APPLY(NZ) k_apply<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, o, v, Ap, st, part_apply)
         if (g0.nz <= 4) TH_APPLY(4);
         else if (g0.nz <= 8) TH_APPLY(8);
         else if (g0.nz <= 12) TH_APPLY(12);
@@ -469,9 +476,9 @@ struct Solver final : SolverBase {
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
-    int launch_apply() {  // with alpha in the state (Jacobi path and tests)
+    int launch_apply() {  // Ap = A p with alpha in the state (Jacobi path and tests)
         int rc;
-        if ((rc = launch_apply_kernel())) return rc;
+        if ((rc = launch_apply_kernel(p))) return rc;
         return finalize<1, FIN_ALPHA>(nblk_stencil, part_apply);
     }
 
@@ -487,7 +494,7 @@ struct Solver final : SolverBase {
             if ((rc = vcycle(0, Cons{part_res, (int)nblk_stencil, CONS_START}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
-            k_pupdate<T, 1><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
+            k_pupdate<T, 1, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, pm, nullptr, st, part_dot, (int)vb);
             HIPTRY(hipGetLastError());
             return TH_OK;
         }
@@ -504,18 +511,19 @@ struct Solver final : SolverBase {
         const unsigned vb = vec_blocks(n0);
         int rc;
         if (mg()) {
-            if ((rc = launch_apply_kernel())) return rc;
+            if ((rc = launch_apply_kernel(pm))) return rc;
             if (opt.beta == TH_BETA_FLEXIBLE)
-                k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, L[0].x, nullptr, nullptr,
-                                                                   st, part_upd, part_apply, (int)nblk_stencil);
+                k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x, nullptr,
+                                                                   nullptr, st, part_upd, part_apply,
+                                                                   (int)nblk_stencil);
             else
-                k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, L[0].b, nullptr, nullptr, nullptr, st,
-                                                              part_upd, part_apply, (int)nblk_stencil);
+                k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr, nullptr,
+                                                              st, part_upd, part_apply, (int)nblk_stencil);
             HIPTRY(hipGetLastError());
             if ((rc = vcycle(0, Cons{part_upd, (int)vb, CONS_ITER}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
-            k_pupdate<T, 0><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, p, st, part_dot, (int)vb);
+            k_pupdate<T, 0, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, pm, x, st, part_dot, (int)vb);
             HIPTRY(hipGetLastError());
             return TH_OK;
         }
@@ -524,7 +532,7 @@ struct Solver final : SolverBase {
                                                           nullptr, 0);
         HIPTRY(hipGetLastError());
         if ((rc = finalize<2, FIN_UPDATE_JACOBI>(vb, part_upd))) return rc;
-        k_pupdate<double, 0><<<vb, kVecThreads, 0, s>>>(n0, z, p, st, nullptr, 0);
+        k_pupdate<double, 0, double><<<vb, kVecThreads, 0, s>>>(n0, z, p, nullptr, st, nullptr, 0);
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
@@ -584,7 +592,7 @@ struct Solver final : SolverBase {
         HIPTRY(hipGetLastError());
         if ((rc = run_start()) || (rc = fetch_state())) return rc;
         if ((rc = converge(maxit, 1))) return rc;
-        k_finish<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, st);
+        k_finish<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, pm, st);
         HIPTRY(hipGetLastError());
         if (out) {
             out->iters = h_st->iters;
@@ -602,6 +610,8 @@ struct Solver final : SolverBase {
         int rc;
         if (maxit > 0 && (rc = iterate(maxit, first))) return rc;
         for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
+            k_finish<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, pm, st);  // x complete
+            HIPTRY(hipGetLastError());
             k_reopen<<<1, 1, 0, s>>>(st);
             HIPTRY(hipGetLastError());
             if ((rc = run_start()) || (rc = fetch_state())) return rc;
@@ -635,7 +645,7 @@ struct Solver final : SolverBase {
     }
     int enqueue_step_tail() {
         const unsigned vb = vec_blocks(n0);
-        k_finish<<<vb, kVecThreads, 0, s>>>(n0, x, st);
+        k_finish<T><<<vb, kVecThreads, 0, s>>>(n0, x, pm, st);
         HIPTRY(hipGetLastError());
         k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
         HIPTRY(hipGetLastError());
@@ -812,9 +822,8 @@ struct Solver final : SolverBase {
             switch (which) {
                 case TH_K_SWEEP:
                     return launch_sweep<SW_PLAIN>(0, 0);
-                case TH_K_APPLY: {
-                    return launch_apply_kernel();
-                }
+                case TH_K_APPLY:
+                    return mg() ? launch_apply_kernel(pm) : launch_apply_kernel(p);
                 case TH_K_RESTRICT:
                     return launch_restrict(0);
                 case TH_K_COPY:
@@ -829,7 +838,7 @@ struct Solver final : SolverBase {
         };
         switch (which) {
             case TH_K_SWEEP: *bytes = 4.5 * sz * N; break;                  // b, x(other), gE, gN, gU, 1/pivot; x
-            case TH_K_APPLY: *bytes = (16.0 + 3.0 * sz) * N; break;         // p, Ap; gE, gN, gU
+            case TH_K_APPLY: *bytes = (8.0 + (mg() ? sz : 8.0) + 3.0 * sz) * N; break;  // Ap, p; gE, gN, gU
             case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN; residual form: b, x, gE, gN, gU; coarse b
                 *bytes = (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
                 break;
