@@ -24,7 +24,7 @@ enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 constexpr int kTailThreads = 512;
 constexpr int kMaxTail = 8;
 constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
-constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 4 or 8 KB static)
+constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 5 or 9 KB static)
 constexpr int kTailLdsThreads = 1024;           // plane-parallel sweeps: 64 / LP columns per wave
 
 // Coarse parents of the 4 lateral neighbours of column (i, j) (SW_PROLONG).
@@ -117,9 +117,10 @@ __device__ __forceinline__ void sweep_column(const Geo& g, const T* __restrict__
 // above is latency-bound (one thread issuing ~11 loads per plane): LP lanes
 // per column, one per plane, 256 / LP columns per 256-thread block (a column
 // never spans two wavefronts). Each lane gathers its plane's right-hand
-// side, 1/pivot and gU into LDS; the column's first lane runs the Thomas
-// recurrence from LDS (the same arithmetic in the same order as
-// sweep_column) and every lane writes its plane back.
+// side, 1/pivot and gU into LDS; thread k < 256 / LP then runs column k's
+// Thomas recurrence from LDS (the same arithmetic in the same order as
+// sweep_column): one wave issues the serial part, not every wave for its
+// own few columns. Every lane writes its plane back.
 constexpr int kPPThreads = 256;
 constexpr long kPPCols = 8192;  // levels with at most this many columns per colour sweep plane-parallel
 __host__ __device__ constexpr int pp_lanes(int nz) { return nz <= 4 ? 4 : nz <= 8 ? 8 : 16; }
@@ -128,7 +129,8 @@ __device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const 
                                               const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
                                               const T* __restrict__ xc, State* st,
                                               const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
-    __shared__ T sh_s[kPPThreads], sh_i[kPPThreads], sh_u[kPPThreads];
+    __shared__ __align__(16) T sh_s[kPPThreads], sh_i[kPPThreads], sh_u[kPPThreads];
+    __shared__ int sh_ok[kPPThreads / 4];
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
     constexpr int CPB = kPPThreads / LP;
@@ -147,18 +149,20 @@ __device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const 
         sh_i[tid] = finv[id];
         sh_u[tid] = o.gU[id];
     }
+    if (l == 0) sh_ok[cb] = colok;
     __syncthreads();
-    if (colok && l == 0) {
+    if (tid < CPB && sh_ok[tid]) {
+        const int o0 = tid * LP;
         T s[LP], inv[LP], gu[LP];
 #pragma unroll
         for (int k = 0; k < LP; ++k) {
-            s[k] = sh_s[tid + k];
-            inv[k] = sh_i[tid + k];
-            gu[k] = sh_u[tid + k];
+            s[k] = sh_s[o0 + k];
+            inv[k] = sh_i[o0 + k];
+            gu[k] = sh_u[o0 + k];
         }
         column_thomas<T, LP>(g.nz, s, inv, gu);
 #pragma unroll
-        for (int k = 0; k < LP; ++k) sh_s[tid + k] = s[k];
+        for (int k = 0; k < LP; ++k) sh_s[o0 + k] = s[k];
     }
     __syncthreads();
     if (ok) x[(long)l * g.plane + c.c0] = sh_s[tid];
@@ -415,7 +419,8 @@ __device__ __forceinline__ LdsLevel<T> lds_level(const Tail<T>& t, T* sm, int k)
 }
 
 template <class T, int LP, int MODE>
-__device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s) {
+__device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s,
+                                              int* scr_c) {
     constexpr int CPP = kTailLdsThreads / LP;  // columns per pass
     const int tid = threadIdx.x, l = tid % LP;
     const Geo& g = F.g;
@@ -432,22 +437,26 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
             const Parents pa = MODE == SW_PROLONG ? parents(C.g, c, i, j) : Parents{0, 0, 0, 0};
             scr_s[tid] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
         }
+        if (l == 0) scr_c[tid / LP] = colok ? c.c0 : -1;
         __syncthreads();
-        if (colok && l == 0) {
+        // Thread k < CPP solves column k of the pass (one wave at LP = 16).
+        const int c0 = tid < CPP ? scr_c[tid] : -1;
+        if (c0 >= 0) {
+            T* sc = scr_s + tid * LP;
             if constexpr (sizeof(T) == 4) {
                 T sv[LP], iv[LP], uv[LP];
 #pragma unroll
                 for (int k = 0; k < LP; ++k) {
-                    sv[k] = scr_s[tid + k];
+                    sv[k] = sc[k];
                     if (k < g.nz) {
-                        const long id = (long)k * g.plane + c.c0;
+                        const long id = (long)k * g.plane + c0;
                         iv[k] = F.fi[id];
                         uv[k] = F.gU[id];
                     }
                 }
                 column_thomas<T, LP>(g.nz, sv, iv, uv);
 #pragma unroll
-                for (int k = 0; k < LP; ++k) scr_s[tid + k] = sv[k];
+                for (int k = 0; k < LP; ++k) sc[k] = sv[k];
             } else {
                 // Three FP64 columns of 16 planes exceed a 1024-thread
                 // block's 128 registers: the recurrence (column_thomas's
@@ -455,14 +464,14 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
                 const int nz = g.nz;
                 T prev = 0;
                 for (int k = 0; k < nz; ++k) {
-                    const long id = (long)k * g.plane + c.c0;
+                    const long id = (long)k * g.plane + c0;
                     const T lower = (k > 0) ? -F.gU[id - g.plane] : T(0);
-                    prev = (scr_s[tid + k] - lower * prev) * F.fi[id];
-                    scr_s[tid + k] = prev;
+                    prev = (sc[k] - lower * prev) * F.fi[id];
+                    sc[k] = prev;
                 }
                 for (int k = nz - 2; k >= 0; --k) {
-                    const long id = (long)k * g.plane + c.c0;
-                    scr_s[tid + k] = scr_s[tid + k] - ((-F.gU[id]) * F.fi[id]) * scr_s[tid + k + 1];
+                    const long id = (long)k * g.plane + c0;
+                    sc[k] = sc[k] - ((-F.gU[id]) * F.fi[id]) * sc[k + 1];
                 }
             }
         }
@@ -476,7 +485,8 @@ template <class T, int NZ>
 __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const double* __restrict__ cons_part,
                                               int cons_nblk, int cons_mode, T* sm) {
     constexpr int LP = pp_lanes(NZ);
-    __shared__ T scr_s[kTailLdsThreads];
+    __shared__ __align__(16) T scr_s[kTailLdsThreads];
+    __shared__ int scr_c[kTailLdsThreads / 4];  // pass columns' c0 (-1: none)
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
     for (int k = 0; k < t.n; ++k) {
@@ -495,8 +505,8 @@ __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const
     const int n = t.n;
     for (int k = 0; k + 1 < n; ++k) {
         const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
-        tail_sweep_pp<T, LP, SW_ZERO>(F, 0, C, scr_s);
-        tail_sweep_pp<T, LP, SW_PLAIN>(F, 1, C, scr_s);
+        tail_sweep_pp<T, LP, SW_ZERO>(F, 0, C, scr_s, scr_c);
+        tail_sweep_pp<T, LP, SW_PLAIN>(F, 1, C, scr_s, scr_c);
         const int cells = C.g.nx * C.g.ny * C.g.nz;
         for (int q = threadIdx.x; q < cells; q += kTailLdsThreads) {
             const int I = q % C.g.nx, rest = q / C.g.nx;
@@ -506,12 +516,12 @@ __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const
     }
     {
         const LdsLevel<T> N = lds_level<T>(t, sm, n - 1);
-        tail_sweep_pp<T, LP, SW_ZERO>(N, 0, N, scr_s);
+        tail_sweep_pp<T, LP, SW_ZERO>(N, 0, N, scr_s, scr_c);
     }
     for (int k = n - 2; k >= 0; --k) {
         const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
-        tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s);
-        tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s);
+        tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s, scr_c);
+        tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s, scr_c);
     }
     const LdsLevel<T> T0 = lds_level<T>(t, sm, 0);
     const long cells0 = t.L[0].g.cells();

@@ -159,6 +159,29 @@ def test_zero_power_and_maxit():
     assert not st2["converged"] and st2["iters"] == 2
 
 
+@pytest.mark.parametrize("k", [1, 2, 5])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stopped_solve_returns_the_iterate(k, precision):
+    """A solve stopped by maxit returns the oracle's k-th iterate: the last
+    update's x += alpha p, which the multigrid path applies one kernel late,
+    is in x when the solve returns."""
+    m = hotspot(48)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-14, maxit=k, mg_double=(precision == "fp64"), flexible=True, fast_restrict=True)
+    s = T.Solver(m, precision=precision, beta="flexible", restriction="lateral")
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-14, k)
+    assert st["iters"] == k and not st["converged"]
+    x = s.get_x()
+    err = np.abs(x - ref["T"]).max() / np.abs(ref["T"]).max()
+    # FP64: only the dot products' summation order differs; mixed: FP32
+    # search direction and V-cycle rounding
+    assert err < (1e-12 if precision == "fp64" else 1e-5)
+    tr = s.true_relres()
+    assert abs(tr - O.true_relres(a, x)) <= 1e-6 * tr
+
+
 @pytest.mark.parametrize("graphs", [True, False])
 def test_transient_vs_oracle(graphs):
     m = hotspot(32)
