@@ -1,0 +1,89 @@
+"""Every plane-count instantiation of the column kernels against the oracle.
+
+The other solver tests use 10 or 16 z-planes. Stacks of 1, 3, 6, 20, 40 and
+64 planes reach the rest: the 4- and 8-lane plane-parallel sweeps and LDS
+tail, the NZ = 4/8/24/32/48/64 column walks and, above 16 planes, the
+L2-resident tail. Bars as in test_gpu_solver: one V-cycle bit-identical to
+the oracle's (same operation order, no FMA contraction on either side), a
+steady solve within one iteration of the oracle's count and 1e-8 of its
+field, true relative residual <= 1e-10.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import thermal_amd as T
+
+pytestmark = pytest.mark.gpu
+
+SI = ("Si", 50e-6, 130.0, 1.66e6)
+BOND = ("bond", 10e-6, 2.0, 1.70e6)
+TIM = ("TIM", 25e-6, 5.0, 2.00e6)
+CU = ("Cu", 1e-3, 400.0, 3.45e6)
+
+# planes per layer of a Si / bond / Si / TIM / Cu stack (or a single Si die)
+STACKS = {
+    1: [SI + (1, 1)],
+    3: [SI + (2, 1), CU + (1, 0)],
+    6: [SI + (1, 1), BOND + (1, 0), SI + (1, 1), TIM + (1, 0), CU + (2, 0)],
+    20: [SI + (6, 1), BOND + (2, 0), SI + (6, 1), TIM + (2, 0), CU + (4, 0)],
+    40: [SI + (12, 1), BOND + (4, 0), SI + (12, 1), TIM + (4, 0), CU + (8, 0)],
+    64: [SI + (20, 1), BOND + (6, 0), SI + (20, 1), TIM + (6, 0), CU + (12, 0)],
+}
+
+
+def stack_model(nz, nx, ny):
+    layers = STACKS[nz]
+    m = T.Model.from_layers(layers, nx, ny)
+    active = [k for k, l in enumerate(layers) if l[5]]
+    m.add_uniform_power(active[0], 20.0)
+    m.add_hotspot(active[0], 3e-3, 4e-3, 1e-3, 3.0)
+    if len(active) > 1:
+        m.add_uniform_power(active[1], 15.0)
+    m.add_tsv_region(active[0], 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
+    m.finalize()
+    assert m.nz == nz
+    return m
+
+
+@pytest.mark.parametrize("nz", sorted(STACKS))
+@pytest.mark.parametrize("precision,mg_double", [("mixed", False), ("fp64", True)])
+@pytest.mark.parametrize("restriction", ["lateral", "residual"])
+def test_vcycle_bitwise(nz, precision, mg_double, restriction):
+    """61 x 45: the fine level runs as separate plane-parallel (nz <= 16) or
+    column-walk (nz > 16) kernels, the coarse levels in a tail kernel."""
+    m = stack_model(nz, 61, 45)
+    s = T.Solver(m, precision=precision, restriction=restriction)
+    o = O.Solver(m.arrays(), "mg", mg_double=mg_double, fast_restrict=(restriction == "lateral"))
+    r = np.random.default_rng(nz).standard_normal(m.n)
+    zg, zo = s.precondition(r), o.precond(r)
+    assert np.array_equal(zg, zo), np.abs(zg - zo).max()
+
+
+@pytest.mark.parametrize("nz", [1, 3, 6])
+def test_vcycle_bitwise_column_walk(nz):
+    """192 x 96: more than 8192 columns per colour, so the fine level runs
+    the column-walk sweep instantiated for 4 and 8 planes."""
+    m = stack_model(nz, 192, 96)
+    s = T.Solver(m, precision="mixed")
+    o = O.Solver(m.arrays(), "mg", mg_double=False, fast_restrict=True)
+    r = np.random.default_rng(100 + nz).standard_normal(m.n)
+    assert np.array_equal(s.precondition(r), o.precond(r))
+
+
+@pytest.mark.parametrize("nz", sorted(STACKS))
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_steady_vs_oracle(nz, precision):
+    m = stack_model(nz, 61, 45)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-10, mg_double=(precision == "fp64"), flexible=True, fast_restrict=True)
+    s = T.Solver(m, precision=precision, true_residual=True)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 200)
+    x = s.get_x()
+    assert st["converged"] and abs(st["iters"] - ref["iters"]) <= 1
+    assert np.abs(x - ref["T"]).max() < 1e-8 * ref["T"].max()
+    assert O.true_relres(a, x) <= 1.5e-10
+    P = m.total_power()
+    assert abs(O.heat_out(a, x) - P) / P < 1e-6
