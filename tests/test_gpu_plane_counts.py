@@ -82,8 +82,47 @@ def test_steady_vs_oracle(nz, precision):
     s.set_rhs_steady()
     st = s.solve(1e-10, 200)
     x = s.get_x()
-    assert st["converged"] and abs(st["iters"] - ref["iters"]) <= 1
+    assert st["converged"] and abs(st["iters"] - ref["iters"]) <= 1, (st, ref["iters"], O.true_relres(a, x))
     assert np.abs(x - ref["T"]).max() < 1e-8 * ref["T"].max()
     assert O.true_relres(a, x) <= 1.5e-10
     P = m.total_power()
     assert abs(O.heat_out(a, x) - P) / P < 1e-6
+
+
+@pytest.mark.parametrize("nz", [1, 3, 6, 16])
+@pytest.mark.parametrize("precision,mg_double", [("mixed", False), ("fp64", True)])
+@pytest.mark.parametrize("n", [(301, 171), (522, 266)])
+def test_vcycle_bitwise_large_levels(nz, precision, mg_double, n):
+    """Levels with more than 8192 columns per colour run the column-walk
+    sweeps: (301, 171) has one such level, (522, 266) two, with odd coarse
+    sizes below them, steady and transient operators."""
+    layers = STACKS[nz] if nz != 16 else [SI + (4, 1), BOND + (2, 0), SI + (4, 1), TIM + (2, 0), CU + (4, 0)]
+    m = T.Model.from_layers(layers, *n)
+    active = [k for k, l in enumerate(layers) if l[5]]
+    m.add_uniform_power(active[0], 20.0)
+    m.add_hotspot(active[0], 3e-3, 4e-3, 1e-3, 3.0)
+    m.add_tsv_region(active[0], 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
+    m.finalize()
+    s = T.Solver(m, precision=precision)
+    o = O.Solver(m.arrays(), "mg", mg_double=mg_double, fast_restrict=True)
+    for dt in (0.0, 1e-3):
+        s.set_dt(dt)
+        o.set_dt(dt)
+        r = np.random.default_rng(7 + nz).standard_normal(m.n)
+        zg, zo = s.precondition(r), o.precond(r)
+        assert np.array_equal(zg, zo), np.abs(zg - zo).max()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_steady_large_levels_vs_oracle(precision):
+    m = T.Model.example_case(300, 212)
+    a = m.arrays()
+    ref = O.steady(a, tol=1e-10, mg_double=(precision == "fp64"), flexible=True, fast_restrict=True)
+    s = T.Solver(m, precision=precision, true_residual=True)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 200)
+    x = s.get_x()
+    assert st["converged"] and abs(st["iters"] - ref["iters"]) <= 1, (st, ref["iters"], O.true_relres(a, x))
+    assert np.abs(x - ref["T"]).max() < 1e-8 * ref["T"].max()
+    assert O.true_relres(a, x) <= 1.5e-10
