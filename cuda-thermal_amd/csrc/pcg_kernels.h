@@ -474,6 +474,81 @@ __global__ __launch_bounds__(kVecThreads) void k_energy(Geo g, Ops<T> o, const d
     block_partials<3, 4u>(v, part);
 }
 
+// ---- die map: colour image and hottest cell (viewer.cu:53-87) -----------------
+
+// The viewer's palette: five RGB stops, linear in between, s clamped to
+// [0, 1] (the same float operations as the reference's colormap, so the bytes
+// agree).
+__device__ __forceinline__ unsigned palette5(float s) {
+    constexpr float kStops[5][3] = {
+        {0.f, 0.f, 4.f}, {87.f, 16.f, 110.f}, {188.f, 55.f, 84.f}, {249.f, 142.f, 9.f}, {252.f, 255.f, 164.f}};
+    s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+    float f = s * 4.f;
+    int k = (int)f;
+    if (k > 3) k = 3;
+    f -= (float)k;
+    unsigned rgba = 255u << 24;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float lo = kStops[k][c], hi = kStops[k + 1][c];
+        rgba |= (unsigned)(unsigned char)(lo + f * (hi - lo)) << (8 * c);
+    }
+    return rgba;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)v, off, 64), hi = __shfl_xor((unsigned)(v >> 32), off, 64);
+        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Column (i, j): the hottest rise v of planes l0..l1 (at least 0), its colour
+// on [0, rise_hi] as RGBA bytes (rgba may be null), and the block's largest
+// key (bits of v << 32 | j * nx + i): non-negative floats order like their
+// bits, so the largest key is the hottest cell, the last one among equals.
+// Block: 64 x 4 columns (blockDim 256).
+static __global__ __launch_bounds__(256) void k_colorize(Geo g, const double* __restrict__ x, int l0, int l1,
+                                                         float rise_hi, unsigned* __restrict__ rgba,
+                                                         unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long sk[4];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    unsigned long long key = 0;
+    if (i < g.nx && j < g.ny) {
+        const int c0 = g.pos(i, j);
+        float v = 0.f;
+        for (int l = l0; l <= l1; ++l) v = fmaxf(v, (float)x[(long)l * g.plane + c0]);
+        const int cell = j * g.nx + i;
+        if (rgba) rgba[cell] = palette5(v / rise_hi);
+        key = ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)cell;
+    }
+    key = wave_max_u64(key);
+    if ((threadIdx.x & 63) == 0) sk[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sk[0];
+        for (int w = 1; w < 4; ++w) m = sk[w] > m ? sk[w] : m;
+        part[blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
+}
+// Largest of n keys (one block of 256 threads).
+static __global__ __launch_bounds__(256) void k_max_key(const unsigned long long* __restrict__ part, int n,
+                                                        unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sk[4];
+    unsigned long long m = 0;
+    for (int k = threadIdx.x; k < n; k += 256) m = part[k] > m ? part[k] : m;
+    m = wave_max_u64(m);
+    if ((threadIdx.x & 63) == 0) sk[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) m = sk[w] > m ? sk[w] : m;
+        out[0] = m > sk[0] ? m : sk[0];
+    }
+}
+
 // ---- initial guess from a coarse field ----------------------------------------
 
 // Plane-to-channel map of the lift: plane l of the solver's grid takes

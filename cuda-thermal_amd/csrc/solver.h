@@ -93,6 +93,8 @@ struct SolverBase {
     virtual int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
                              int* used, double* rp, double* rl) = 0;
     virtual int nz() const = 0;
+    virtual int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
+                         int* hot_cell) = 0;
 };
 
 template <class T>
@@ -760,6 +762,41 @@ struct Solver final : SolverBase {
     int nz() const override { return g0.nz; }
 
     int true_relres(double* out) override { return relres_of(x, out); }
+
+    // Die map of x (viewer.cu:53-87): RGBA image (host or device memory, or
+    // none) and the hottest cell of planes l0..l1.
+    unsigned* map_rgba = nullptr;
+    unsigned long long *map_part = nullptr, *map_key = nullptr;
+    int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
+                 int* hot_cell) override {
+        if (l0 < 0 || l1 < l0 || l1 >= g0.nz) return fail(TH_ERR_ARG, "colorize: planes out of range");
+        if (!(rise_hi > 0.f) || !std::isfinite(rise_hi)) return fail(TH_ERR_ARG, "colorize: rise_hi must be > 0");
+        const dim3 grid((unsigned)((g0.nx + 63) / 64), (unsigned)((g0.ny + 3) / 4));
+        const int nb = (int)(grid.x * grid.y);
+        int rc;
+        if (!map_part && ((rc = dalloc(&map_part, nb)) || (rc = dalloc(&map_key, 1)))) return rc;
+        unsigned* dst = nullptr;
+        if (rgba && device_rgba) dst = (unsigned*)rgba;
+        else if (rgba) {
+            if (!map_rgba && (rc = dalloc(&map_rgba, (size_t)g0.nx * g0.ny))) return rc;
+            dst = map_rgba;
+        }
+        k_colorize<<<grid, 256, 0, s>>>(g0, x, l0, l1, rise_hi, dst, map_part);
+        HIPTRY(hipGetLastError());
+        k_max_key<<<1, 256, 0, s>>>(map_part, nb, map_key);
+        HIPTRY(hipGetLastError());
+        unsigned long long key = 0;
+        HIPTRY(hipMemcpyAsync(&key, map_key, sizeof(key), hipMemcpyDeviceToHost, s));
+        if (rgba && !device_rgba)
+            HIPTRY(hipMemcpyAsync(rgba, map_rgba, (size_t)g0.nx * g0.ny * 4, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        const unsigned bits = (unsigned)(key >> 32);
+        float v;
+        std::memcpy(&v, &bits, 4);
+        if (hot_rise) *hot_rise = v;
+        if (hot_cell) *hot_cell = (int)(key & 0xffffffffu);
+        return TH_OK;
+    }
     // ||P - A v|| / ||P|| of the steady operator (flux form)
     int relres_of(const double* v, double* out) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);

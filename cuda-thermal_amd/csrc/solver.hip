@@ -117,6 +117,17 @@ int th_solver_heat_out(th_solver* s, double* out) {
     return s->impl->energy(nullptr, out, nullptr);
 }
 int th_solver_nlevels(th_solver* s) { return s ? s->impl->nlevels() : -1; }
+int th_solver_colorize(th_solver* s, int l0, int l1, float rise_hi, unsigned char* rgba, float* hot_rise,
+                       int* hot_cell) {
+    TH_S(s);
+    return s->impl->colorize(l0, l1, rise_hi, rgba, false, hot_rise, hot_cell);
+}
+int th_solver_colorize_device(th_solver* s, int l0, int l1, float rise_hi, unsigned char* d_rgba, float* hot_rise,
+                              int* hot_cell) {
+    TH_S(s);
+    if (!d_rgba) return fail(TH_ERR_ARG, "null device image");
+    return s->impl->colorize(l0, l1, rise_hi, d_rgba, true, hot_rise, hot_cell);
+}
 int th_solver_precondition(th_solver* s, const double* r, double* z) {
     TH_S(s);
     if (!r || !z) return fail(TH_ERR_ARG, "null argument");

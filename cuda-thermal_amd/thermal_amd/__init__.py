@@ -123,6 +123,8 @@ SIGNATURES = {
     "th_solver_true_relres": (C.c_int, [_vp, _dp]),
     "th_solver_heat_out": (C.c_int, [_vp, _dp]),
     "th_solver_nlevels": (C.c_int, [_vp]),
+    "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
+    "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_precondition": (C.c_int, [_vp, _dp, _dp]),
     "th_solver_apply_operator": (C.c_int, [_vp, _dp, _dp]),
     "th_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
@@ -419,6 +421,17 @@ class Solver:
 
     def nlevels(self):
         return lib().th_solver_nlevels(self.h)
+
+    def colorize(self, l0, l1, rise_hi, image=True):
+        """Die map of the current T (viewer.cu:68-87): the hottest rise of
+        planes l0..l1 per column, coloured on [0, rise_hi] as an (ny, nx, 4)
+        RGBA uint8 image (None with image=False), and the hottest cell:
+        (rise, (i, j))."""
+        img = np.empty((self.ny, self.nx, 4), np.uint8) if image else None
+        v, cell = C.c_float(), C.c_int()
+        ptr = img.ctypes.data_as(_vp) if image else None
+        _check(lib().th_solver_colorize(self.h, l0, l1, rise_hi, ptr, C.byref(v), C.byref(cell)))
+        return img, v.value, (cell.value % self.nx, cell.value // self.nx)
 
     def precondition(self, r):
         """z = M^-1 r for the current dt (one V-cycle, or Jacobi)."""

@@ -162,6 +162,19 @@ TH_API int th_solver_true_relres(th_solver* s, double* out);
 /* heat_out (stack3d.h:254) at the current x, on the device. */
 TH_API int th_solver_heat_out(th_solver* s, double* out);
 TH_API int th_solver_nlevels(th_solver* s);
+/* Die map of the current x (the viewer's k_colorize, viewer.cu:68-87): for
+ * each column (i, j) the hottest temperature rise v over planes l0..l1 (at
+ * least 0), coloured on [0, rise_hi] with the viewer's five-stop palette
+ * (colormap, viewer.cu:53-66) into rgba[4 * (j * nx + i) + 0..3] (R, G, B,
+ * 255; host memory, nx * ny * 4 bytes, or NULL for no image); and the hottest
+ * cell of those planes, *hot_rise = its v and *hot_cell = j * nx + i (the
+ * largest index among equal values). rise_hi must be > 0. */
+TH_API int th_solver_colorize(th_solver* s, int l0, int l1, float rise_hi, unsigned char* rgba, float* hot_rise,
+                              int* hot_cell);
+/* The same with the image written to device memory (d_rgba, nx * ny * 4
+ * bytes, e.g. a buffer a display maps), as the viewer writes its PBO. */
+TH_API int th_solver_colorize_device(th_solver* s, int l0, int l1, float rise_hi, unsigned char* d_rgba,
+                                     float* hot_rise, int* hot_cell);
 /* z = M^-1 r: one preconditioner application (the multigrid V-cycle in the
  * solver's precision, or Jacobi) for the current dt, host arrays in the
  * natural layout (CpuPcg::precondition, pcg3d_cpu.cpp:115-124). */
