@@ -70,38 +70,76 @@ def run_mode(model, name, steps, warmup, **opts):
     return s, x, res
 
 
-def cpu_baseline(folder, n, iters):
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(folder, n, iters, transient_iters=887):
     """The reference's CPU MG-PCG (oracle/_ref, built from the reference
-    sources) or else the oracle's restatement, single-threaded, on the same
-    case folder: a bounded sample of 2 PCG iterations, scaled to a solve of
-    `iters` iterations (plus the initial residual and V-cycle)."""
+    sources) or else the oracle's restatement, single-threaded and pinned to
+    one core, on the same inputs: config 2 (this case folder) as a bounded
+    sample of 2 PCG iterations scaled to a solve of `iters` iterations plus
+    the start; config 0 (128x128x10 hotspot stack) solved in full; config 1
+    (800x800x10, 100 backward-Euler steps) estimated from a 2-iteration
+    sample and the transient's iteration count."""
     import oracle as O
 
     sample_it = 2
-    t0 = time.perf_counter()
-    if O.ref_available():
-        kind = "reference"
-        m = O.RefModel.case(folder, n)
-        r = m.cpu_pcg(TOL, maxit=sample_it, precond="mg")
-        solve_ms = r["solve_ms"]
-    else:
-        kind = "port"
-        import thermal_amd as TT
+    info = cpu_info()
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})  # this thread: the CPU solves run in it
+    try:
+        t0 = time.perf_counter()
+        if O.ref_available():
+            kind = "reference"
+            m = O.RefModel.case(folder, n)
+            solve_ms = m.cpu_pcg(TOL, maxit=sample_it, precond="mg")["solve_ms"]
+        else:
+            kind = "port"
+            import thermal_amd as TT
 
-        a = TT.Model.load_case(folder, n).arrays()
-        s = O.Solver(a, "mg")
-        s.set_x_zero()
-        s.set_rhs_steady()
-        t1 = time.perf_counter()
-        s.solve(TOL, sample_it)
-        solve_ms = (time.perf_counter() - t1) * 1e3
-    wall = time.perf_counter() - t0
-    per_it = solve_ms / (sample_it + 1)
+            a = TT.Model.load_case(folder, n).arrays()
+            s = O.Solver(a, "mg")
+            s.set_x_zero()
+            s.set_rhs_steady()
+            t1 = time.perf_counter()
+            s.solve(TOL, sample_it)
+            solve_ms = (time.perf_counter() - t1) * 1e3
+        wall = time.perf_counter() - t0
+        per_it = solve_ms / (sample_it + 1)
+        extra = {}
+        if kind == "reference":
+            def hotspot(nn, at):
+                h = O.RefModel.default(nn)
+                h.add_uniform_power(0, 22.0)
+                h.add_hotspot(0, at, at, 1e-3, 3.0)
+                h.add_uniform_power(2, 25.0)
+                h.finalize()
+                return h
+
+            r0 = hotspot(128, 5e-3).cpu_pcg(TOL, maxit=500, precond="mg")
+            extra["config0_128x128x10"] = dict(ms=round(r0["solve_ms"], 1), iters=int(r0["iters"]))
+            r1 = hotspot(800, 5e-3).cpu_pcg(TOL, maxit=sample_it, precond="mg")
+            it1 = r1["solve_ms"] / (sample_it + 1)
+            extra["config1_800x800x10_100_steps_estimate_s"] = round(it1 * (transient_iters + 100) / 1e3, 1)
+    finally:
+        os.sched_setaffinity(0, old)
     value = per_it * (iters + 1)
     return dict(value=round(value, 1), unit="ms per solve (estimated)", cores=1, kind=kind,
                 sample=(f"{n}x{n}x16 config, {sample_it} MG-PCG iterations ({solve_ms:.0f} ms, {wall:.1f} s wall "
-                        f"incl. model and level setup) on 1 core, scaled to {iters} iterations + start"),
-                ms_per_iteration=round(per_it, 1))
+                        f"incl. model and level setup) on 1 core (pinned to cpu {core}), scaled to {iters} "
+                        f"iterations + start"),
+                ms_per_iteration=round(per_it, 1), cpu=info, **extra)
 
 
 def transient_leg():
@@ -278,7 +316,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("  cpu baseline (bounded sample) ...")
-        cpu = cpu_baseline(folder, n, r64["iters"])
+        cpu = cpu_baseline(folder, n, r64["iters"], tr["fp64"]["iterations"] if tr else 887)
         log(f"  cpu: {cpu}")
 
     head = learned if learned else rmx
