@@ -75,56 +75,66 @@ __global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double
 }
 
 // Ap = (A + C/dt) p; partial p.Ap.
+// One column of Ap and its p.Ap terms. TR: the C/dt term is in the diagonal
+// (steady solves never load the capacities).
+template <class T, int NZ, class PT, bool TR>
+__device__ __forceinline__ double apply_column(const Geo& g, const Ops<T>& o, const PT* __restrict__ p,
+                                               double* __restrict__ Ap, const Col& c, double inv_dt) {
+    const int nz = g.nz;
+    const T* __restrict__ gE = o.gE;
+    const T* __restrict__ gN = o.gN;
+    const T* __restrict__ gU = o.gU;
+    const T* __restrict__ cap = o.cap;
+    double v = 0.0;
+    double pb = 0.0, pc = (double)p[c.c0], gdn = (double)o.gB[c.c0];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < nz) {
+            const long base = (long)l * g.plane, id = base + c.c0;
+            const double pu = (l < nz - 1) ? (double)p[id + g.plane] : 0.0;
+            const double gu = (double)gU[id];
+            const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id];
+            const double Gs = (double)gN[base + c.ps], Gn = (double)gN[id];
+            const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
+                         xn = (double)p[base + c.pn];
+            double d = 0.0, s = 0.0;
+            if (c.hw) d += Gw;
+            if (c.he) d += Ge;
+            if (c.hs) d += Gs;
+            if (c.hn) d += Gn;
+            d += gu;
+            d += gdn;  // gU of the plane below, or gB on the bottom plane
+            if (TR) d = d + (double)cap[id] * inv_dt;
+            if (c.hw) s += Gw * xw;
+            if (c.he) s += Ge * xe;
+            if (c.hs) s += Gs * xs;
+            if (c.hn) s += Gn * xn;
+            if (l > 0) s += gdn * pb;
+            if (l < nz - 1) s += gu * pu;
+            const double a = d * pc - s;
+            Ap[id] = a;
+            v += pc * a;
+            pb = pc;
+            pc = pu;
+            gdn = gu;
+        }
+    }
+    return v;
+}
 template <class T, int NZ, class PT = double>
 __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops<T>& o, const PT* __restrict__ p,
                                            double* __restrict__ Ap, const State* __restrict__ st,
                                            double* __restrict__ part) {
-    const int nz = g.nz;
     if (st->done) return;
     const int q = B.x * blockDim.x + threadIdx.x;
     const int j = B.y * blockDim.y + threadIdx.y;
     double v[1] = {0.0};
     const double inv_dt = st->inv_dt;
-    const T* __restrict__ gE = o.gE;
-    const T* __restrict__ gN = o.gN;
-    const T* __restrict__ gU = o.gU;
-    const T* __restrict__ cap = o.cap;
     int i;
     if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
         const Col c = make_col(g, i, j);
-        double pb = 0.0, pc = (double)p[c.c0], gdn = (double)o.gB[c.c0];
-#pragma unroll
-        for (int l = 0; l < NZ; ++l) {
-            if (l < nz) {
-                const long base = (long)l * g.plane, id = base + c.c0;
-                const double pu = (l < nz - 1) ? (double)p[id + g.plane] : 0.0;
-                const double gu = (double)gU[id];
-                const double Gw = (double)gE[base + c.pw], Ge = (double)gE[id];
-                const double Gs = (double)gN[base + c.ps], Gn = (double)gN[id];
-                const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
-                             xn = (double)p[base + c.pn];
-                double d = 0.0, s = 0.0;
-                if (c.hw) d += Gw;
-                if (c.he) d += Ge;
-                if (c.hs) d += Gs;
-                if (c.hn) d += Gn;
-                d += gu;
-                d += gdn;  // gU of the plane below, or gB on the bottom plane
-                if (inv_dt != 0.0) d = d + (double)cap[id] * inv_dt;
-                if (c.hw) s += Gw * xw;
-                if (c.he) s += Ge * xe;
-                if (c.hs) s += Gs * xs;
-                if (c.hn) s += Gn * xn;
-                if (l > 0) s += gdn * pb;
-                if (l < nz - 1) s += gu * pu;
-                const double a = d * pc - s;
-                Ap[id] = a;
-                v[0] += pc * a;
-                pb = pc;
-                pc = pu;
-                gdn = gu;
-            }
-        }
+        v[0] = inv_dt != 0.0 ? apply_column<T, NZ, PT, true>(g, o, p, Ap, c, inv_dt)
+                             : apply_column<T, NZ, PT, false>(g, o, p, Ap, c, inv_dt);
     }
     block_partials<1>(v, part, B.flat());
 }
