@@ -277,7 +277,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single solve "
-              "does not shard across GPUs here, see DESIGN.md)", file=sys.stderr)
+              "does not shard across GPUs here)", file=sys.stderr)
         sys.exit(2)
     if args.kernels_only:
         kernels_only(args.n)

@@ -9,7 +9,7 @@ Random numbers come from splitmix64 (Steele, Lea and Flood, "Fast
 Splittable Pseudorandom Number Generators", OOPSLA 2014): a 64-bit state
 advanced by 0x9E3779B97F4A7C15 and mixed by two xor-shift-multiply rounds.
 
-Choices the benchmark description leaves open (stated here and in DESIGN.md):
+Choices the workload descriptions leave open:
 * Background power: 10% of each die's 25 W is spread uniformly; the 12
   blocks are rescaled to the other 90% (22.5 W).
 * TSV clusters: 16 x 16 cells of the 1024 x 1024 grid (0.15625 mm squares on

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
 // calls this file; only tests/, __graft_entry__.smoke() and the cpu_baseline
-// leg of bench.py use it, and only as the checker / the CPU baseline.
+// leg of bench.py use it, and only as the test oracle / the CPU baseline.
 //
 // It follows the reference's CPU path step by step, in the same operation
 // order, so that a float-for-float comparison with the reference (the golden

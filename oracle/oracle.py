@@ -2,7 +2,7 @@
 (_ref/libref.so, built from the reference's own CPU sources).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or as the CPU baseline. The
+cpu_baseline leg of bench.py, as the test oracle or as the CPU baseline. The
 product library never loads it.
 """
 import ctypes as C

@@ -1,8 +1,8 @@
 """GPU parity tests: the HIP solver through the C ABI against the oracle (the
 CPU restatement of the reference's pcg3d_cpu path) on the same inputs.
 
-Bars (the reference's own checks, main3d.cu:94-282, tightened to the
-north-star numbers): solution within 5e-4 C max-abs of the FP64 oracle;
+Bars (the reference's own checks, main3d.cu:94-282, tightened): solution
+within 5e-4 C max-abs of the FP64 oracle;
 true relative residual <= 1e-10; energy balance to 1e-6; the exact 1D
 resistor chain under uniform power; FP64 operator apply and the FP32 / FP64
 V-cycle bit-identical to the oracle's (same operation order, no FMA

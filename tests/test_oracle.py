@@ -1,5 +1,5 @@
 """Pins the oracle (oracle/oracle.cpp, the CPU restatement of the reference's
-MG-PCG path) before it is trusted as the GPU's checker:
+MG-PCG path) before the GPU's results are compared with it:
   * against the golden fixtures written by the reference's own CPU build
     (tests/golden/make_golden.py), bit for bit where the reference is
     deterministic;
