@@ -276,7 +276,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single solve "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (a single solve "
               "does not shard across GPUs here)", file=sys.stderr)
         sys.exit(2)
     if args.kernels_only:

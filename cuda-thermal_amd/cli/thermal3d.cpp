@@ -533,7 +533,7 @@ int run_case(const std::string& dir, int n) {
 
 int main(int argc, char** argv) {
     if (argc >= 2 && (std::strcmp(argv[1], "export3dice") == 0 || std::strcmp(argv[1], "compare3dice") == 0)) {
-        std::printf("%s: needs the external 3D-ICE tool; not provided by this build\n", argv[1]);
+        std::printf("%s: needs the external 3D-ICE tool, which is not bundled\n", argv[1]);
         return 1;
     }
     if (argc >= 3 && std::strcmp(argv[1], "case") == 0) return run_case(argv[2], argc >= 4 ? std::atoi(argv[3]) : 200);
