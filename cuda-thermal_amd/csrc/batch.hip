@@ -260,6 +260,7 @@ struct Batch {
         for (int i = 0; i < n; ++i) {
             Solver<float>* x = new Solver<float>();
             S.push_back(x);
+            x->allow_types = false;  // the batched kernels read the per-cell arrays
             int rc = x->init(*m[i], io);
             if (rc) return rc;
             k_set_flexible<<<1, 1, 0, x->s>>>(x->st, 1);

@@ -21,6 +21,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "coltypes.h"
+
 #include <cstdint>
 
 namespace th {
@@ -104,6 +106,23 @@ __device__ __forceinline__ Blk hw_blk() {
 template <class T>
 struct Ops {
     const T *gE, *gN, *gU, *gB, *cap;
+};
+
+// Column types of the fine level (levels.h: column_types): the type of each
+// plane slot and the types' faces, table[type][plane][field]: one plane's
+// faces are one aligned row, read with wide loads.
+template <class T>
+struct alignas(sizeof(T) * kTypeFields) TypeRow {
+    T v[kTypeFields];
+};
+template <class T>
+struct Typed {
+    const unsigned short* __restrict__ ty;
+    const T* __restrict__ tab;
+    int nzp;
+    __device__ __forceinline__ TypeRow<T> row(int t, int l) const {
+        return reinterpret_cast<const TypeRow<T>*>(tab)[(long)t * nzp + l];
+    }
 };
 
 // ---- reductions -------------------------------------------------------------

@@ -2,6 +2,9 @@
 #include "levels.h"
 
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
 
 namespace th {
 
@@ -84,6 +87,112 @@ std::vector<HostLevel> build_hierarchy(int nx, int ny, int nz, double dx, double
     f.cap.assign(cap, cap + n);
     while (L.back().nx > 1 || L.back().ny > 1) L.push_back(coarsen(L.back()));
     return L;
+}
+
+
+namespace {
+inline uint64_t mix64(uint64_t h, uint64_t v) {  // splitmix64 finaliser of h ^ v
+    uint64_t z = (h ^ v) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+}  // namespace
+
+ColumnTypes column_types(int nx, int ny, int nz, const float* gE, const float* gN, const float* gU,
+                         const float* gB, const float* cap, int max_types) {
+    ColumnTypes out;
+    const long cols = (long)nx * ny, plane = cols;
+    // field f of every column on plane l (west / south faces are 0 where the
+    // neighbour is absent), into buf
+    std::vector<float> buf(cols);
+    auto plane_values = [&](int f, int l) -> const float* {
+        const long o = (long)l * plane;
+        if (f == TF_W) {
+            for (int j = 0; j < ny; ++j) {
+                const long r = (long)j * nx;
+                buf[r] = 0.f;
+                std::memcpy(&buf[r + 1], gE + o + r, (size_t)(nx - 1) * sizeof(float));
+            }
+            return buf.data();
+        }
+        if (f == TF_S) {
+            std::fill(buf.begin(), buf.begin() + nx, 0.f);
+            std::memcpy(&buf[nx], gN + o, (size_t)(cols - nx) * sizeof(float));
+            return buf.data();
+        }
+        return (f == TF_E ? gE : f == TF_N ? gN : f == TF_U ? gU : cap) + o;
+    };
+    auto value = [&](long c, int f, int l) -> float {  // one value (representatives only)
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const long id = (long)l * plane + c;
+        switch (f) {
+            case TF_E: return gE[id];
+            case TF_N: return gN[id];
+            case TF_U: return gU[id];
+            case TF_C: return cap[id];
+            case TF_W: return i > 0 ? gE[id - 1] : 0.f;
+            case TF_S: return j > 0 ? gN[id - nx] : 0.f;
+            default: return 0.f;
+        }
+    };
+    auto flags_of = [&](long c) -> unsigned char {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        return (unsigned char)((i > 0) | ((i < nx - 1) << 1) | ((j > 0) << 2) | ((j < ny - 1) << 3));
+    };
+    std::vector<uint64_t> h1(cols);
+    for (long c = 0; c < cols; ++c) h1[c] = mix64(0x243F6A8885A308D3ull, flags_of(c) | ((uint64_t)fbits(gB[c]) << 8));
+    for (int l = 0; l < nz; ++l)  // plane by plane: every array is read in order
+        for (int f = TF_E; f <= TF_S; ++f) {
+            const float* vals = plane_values(f, l);
+            const uint64_t tag = (uint64_t)(f * 64 + l) << 32;
+            for (long c = 0; c < cols; ++c) h1[c] = mix64(h1[c], fbits(vals[c]) | tag);
+        }
+    std::unordered_map<uint64_t, int> ids;
+    std::vector<long> rep;
+    out.type.resize(cols);
+    for (long c = 0; c < cols; ++c) {
+        auto it = ids.find(h1[c]);
+        int t;
+        if (it == ids.end()) {
+            t = (int)rep.size();
+            if (t >= max_types) return ColumnTypes{};
+            ids.emplace(h1[c], t);
+            rep.push_back(c);
+        } else {
+            t = it->second;  // a hash collision fails the check below
+        }
+        out.type[c] = (unsigned short)t;
+    }
+    const int nt = (int)rep.size();
+    out.tab.assign((size_t)nt * kTypeFields * nz, 0.f);
+    out.flags.resize(nt);
+    for (int t = 0; t < nt; ++t) {
+        out.flags[t] = flags_of(rep[t]);
+        for (int f = TF_E; f <= TF_S; ++f)
+            for (int l = 0; l < nz; ++l) out.tab[((size_t)t * nz + l) * kTypeFields + f] = value(rep[t], f, l);
+        out.tab[(size_t)t * nz * kTypeFields + TF_B] = gB[rep[t]];
+    }
+    // every column against its type, bit for bit
+    for (long c = 0; c < cols; ++c) {
+        const int t = out.type[c];
+        if (out.flags[t] != flags_of(c) || fbits(out.tab[(size_t)t * nz * kTypeFields + TF_B]) != fbits(gB[c]))
+            return ColumnTypes{};
+    }
+    for (int l = 0; l < nz; ++l)
+        for (int f = TF_E; f <= TF_S; ++f) {
+            const float* vals = plane_values(f, l);
+            const float* col = &out.tab[(size_t)l * kTypeFields + f];  // type t's value: col[t * nz * kTypeFields]
+            for (long c = 0; c < cols; ++c)
+                if (fbits(col[(size_t)out.type[c] * kTypeFields * nz]) != fbits(vals[c])) return ColumnTypes{};
+        }
+    out.ntypes = nt;
+    return out;
 }
 
 }  // namespace th

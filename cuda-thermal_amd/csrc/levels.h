@@ -12,6 +12,8 @@
 #pragma once
 #include <vector>
 
+#include "coltypes.h"
+
 namespace th {
 
 struct HostLevel {
@@ -23,5 +25,24 @@ struct HostLevel {
 // Level 0 is the given fine operator; the last level is 1 x 1.
 std::vector<HostLevel> build_hierarchy(int nx, int ny, int nz, double dx, double dy, const float* gE,
                                        const float* gN, const float* gU, const float* gB, const float* cap);
+
+// Column types of a level. A layered stack repeats a few columns over and
+// over: a column is described by its faces (gE, gN, gU, cap on every plane,
+// the west neighbour's gE and the south neighbour's gN, which are its own
+// west and south faces, and gB) and by which of its 4 lateral neighbours
+// exist. Columns with equal descriptions get one type; the kernels then
+// read a 2-byte type per column and the type's faces from a small table
+// instead of 6 arrays per cell. Types are found by a 64-bit hash per column
+// and every column is then compared, value by value, with its type's
+// description: any mismatch (a hash collision) or more than max_types types
+// returns ntypes = 0, and the caller keeps the per-cell arrays.
+struct ColumnTypes {
+    int ntypes = 0;
+    std::vector<unsigned short> type;  // natural column order j * nx + i
+    std::vector<float> tab;            // [type][plane][field]; TF_FI left 0 (device-computed), TF_B on plane 0
+    std::vector<unsigned char> flags;  // [type]: bit 0 west, 1 east, 2 south, 3 north neighbour present
+};
+ColumnTypes column_types(int nx, int ny, int nz, const float* gE, const float* gN, const float* gU,
+                         const float* gB, const float* cap, int max_types);
 
 }  // namespace th

@@ -314,6 +314,133 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
     }
 }
 
+// ---- typed fine level ----------------------------------------------------------
+// The fine level of a layered stack has few distinct columns (see levels.h:
+// column_types): each plane slot holds a 2-byte type and the faces come from
+// the type's table, so a sweep reads b and x per cell and little else. The
+// values are the per-cell arrays' own (checked bit for bit when the types
+// are built), and every kernel below does its dense counterpart's arithmetic
+// in the same order: results are identical.
+
+// 1/pivot of every type for the current dt (k_factor's arithmetic).
+template <class T>
+__global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const unsigned char* __restrict__ fl, int nt,
+                                                     int nz, int nzp, const State* __restrict__ st) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!st->refactor || t >= nt) return;
+    T* rows = tab + (long)t * nzp * kTypeFields;  // rows[l * kTypeFields + field]
+    const T gB = rows[TF_B];
+    const unsigned fb = fl[t];
+    const bool hw = fb & 1, he = fb & 2, hs = fb & 4, hn = fb & 8;
+    const T inv_dt = (T)st->inv_dt;
+    T cp_prev = 0;
+    for (int l = 0; l < nz; ++l) {
+        const T* r = rows + l * kTypeFields;
+        T d = 0;
+        if (hw) d += r[TF_W];
+        if (he) d += r[TF_E];
+        if (hs) d += r[TF_S];
+        if (hn) d += r[TF_N];
+        d += r[TF_U];
+        d += (l > 0) ? r[TF_U - kTypeFields] : gB;
+        d = d + r[TF_C] * inv_dt;
+        const T lower = (l > 0) ? -r[TF_U - kTypeFields] : T(0);
+        const T upper = (l < nz - 1) ? -r[TF_U] : T(0);
+        const T inv = T(1) / (d - lower * cp_prev);
+        rows[l * kTypeFields + TF_FI] = inv;
+        cp_prev = upper * inv;
+    }
+}
+
+// sweep_column with the faces from the column's type.
+template <class T, int NZ, int MODE>
+__device__ __forceinline__ void sweep_column_typed(const Geo& g, const Typed<T>& y, const T* __restrict__ b,
+                                                   T* __restrict__ x, int i, int j, const Geo& gc,
+                                                   const T* __restrict__ xc) {
+    const int nz = g.nz;
+    const Col c = make_col(g, i, j);
+    const Parents pa = MODE == SW_PROLONG ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
+    const int t = y.ty[c.c0];
+    T s[NZ], inv[NZ], gu[NZ];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < nz) {
+            const long base = (long)l * g.plane, id = base + c.c0;
+            const TypeRow<T> r = y.row(t, l);
+            T v = b[id];
+            if (MODE != SW_ZERO) {  // column_rhs
+                T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
+                if (MODE == SW_PROLONG) {
+                    const long cb = (long)l * gc.plane;
+                    xw = xw + xc[cb + pa.w];
+                    xe = xe + xc[cb + pa.e];
+                    xsv = xsv + xc[cb + pa.s];
+                    xn = xn + xc[cb + pa.n];
+                }
+                const T Gw = r.v[TF_W], Ge = r.v[TF_E], Gs = r.v[TF_S], Gn = r.v[TF_N];
+                T ls = 0;
+                if (c.hw) ls += Gw * xw;
+                if (c.he) ls += Ge * xe;
+                if (c.hs) ls += Gs * xsv;
+                if (c.hn) ls += Gn * xn;
+                v = v + ls;
+            }
+            s[l] = v;
+            inv[l] = r.v[TF_FI];
+            gu[l] = r.v[TF_U];
+        }
+    }
+    column_thomas<T, NZ>(nz, s, inv, gu);
+#pragma unroll
+    for (int l = 0; l < NZ; ++l)
+        if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
+}
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x,
+                                                     int colour, Geo gc, const T* __restrict__ xc, State* st,
+                                                     const double* __restrict__ cons_part, int cons_nblk,
+                                                     int cons_mode) {
+    if (st->done) return;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
+    const int t = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (j >= g.ny) return;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= g.nx) return;
+    sweep_column_typed<T, NZ, MODE>(g, y, b, x, i, j, gc, xc);
+}
+
+// restrict_cell_fast with the red children's faces from their types.
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict_fast_typed(Geo f, Typed<T> y, const T* __restrict__ fx, Geo cg,
+                                                             T* __restrict__ cb, const State* __restrict__ st) {
+    if (st->done) return;
+    const int I = blockIdx.x * kColsX + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= cg.nx || J >= cg.ny) return;
+    const int l = blockIdx.z;
+    const long base = (long)l * f.plane;
+    const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
+    T r[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const Col c = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
+        const TypeRow<T> w = y.row(y.ty[c.c0], l);
+        const T Gw = w.v[TF_W], Ge = w.v[TF_E], Gs = w.v[TF_S], Gn = w.v[TF_N];
+        const T xw = fx[base + c.pw], xe = fx[base + c.pe], xs = fx[base + c.ps], xn = fx[base + c.pn];
+        T s = 0;
+        if (c.hw) s += Gw * xw;
+        if (c.he) s += Ge * xe;
+        if (c.hs) s += Gs * xs;
+        if (c.hn) s += Gn * xn;
+        r[k] = s;
+    }
+    T sum = 0;
+    sum += r[0];
+    if (two) sum += r[1];
+    cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+}
+
 // ---- coarse tail: the lowest levels of the V-cycle in one workgroup --------
 
 template <class T>

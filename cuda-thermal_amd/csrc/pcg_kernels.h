@@ -138,6 +138,66 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
     }
     block_partials<1>(v, part, B.flat());
 }
+// apply_column with the faces from the column's type (fine level).
+template <class T, int NZ, class PT, bool TR>
+__device__ __forceinline__ double apply_column_typed(const Geo& g, const Typed<T>& y, const PT* __restrict__ p,
+                                                     double* __restrict__ Ap, const Col& c, double inv_dt) {
+    const int nz = g.nz;
+    const int t = y.ty[c.c0];
+    double v = 0.0;
+    double pb = 0.0, pc = (double)p[c.c0], gdn = (double)y.row(t, 0).v[TF_B];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        if (l < nz) {
+            const long base = (long)l * g.plane, id = base + c.c0;
+            const TypeRow<T> r = y.row(t, l);
+            const double pu = (l < nz - 1) ? (double)p[id + g.plane] : 0.0;
+            const double gu = (double)r.v[TF_U];
+            const double Gw = (double)r.v[TF_W], Ge = (double)r.v[TF_E];
+            const double Gs = (double)r.v[TF_S], Gn = (double)r.v[TF_N];
+            const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
+                         xn = (double)p[base + c.pn];
+            double d = 0.0, s = 0.0;
+            if (c.hw) d += Gw;
+            if (c.he) d += Ge;
+            if (c.hs) d += Gs;
+            if (c.hn) d += Gn;
+            d += gu;
+            d += gdn;
+            if (TR) d = d + (double)r.v[TF_C] * inv_dt;
+            if (c.hw) s += Gw * xw;
+            if (c.he) s += Ge * xe;
+            if (c.hs) s += Gs * xs;
+            if (c.hn) s += Gn * xn;
+            if (l > 0) s += gdn * pb;
+            if (l < nz - 1) s += gu * pu;
+            const double a = d * pc - s;
+            Ap[id] = a;
+            v += pc * a;
+            pb = pc;
+            pc = pu;
+            gdn = gu;
+        }
+    }
+    return v;
+}
+template <class T, int NZ, class PT>
+__global__ __launch_bounds__(1024) void k_apply_typed(Geo g, Typed<T> y, const PT* __restrict__ p,
+                                                      double* __restrict__ Ap, const State* __restrict__ st,
+                                                      double* __restrict__ part) {
+    if (st->done) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double v[1] = {0.0};
+    const double inv_dt = st->inv_dt;
+    int i;
+    if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
+        const Col c = make_col(g, i, j);
+        v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true>(g, y, p, Ap, c, inv_dt)
+                             : apply_column_typed<T, NZ, PT, false>(g, y, p, Ap, c, inv_dt);
+    }
+    block_partials<1>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+}
 template <class T, int NZ, class PT = double>
 __global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const PT* __restrict__ p,
                                                 double* __restrict__ Ap, const State* __restrict__ st,

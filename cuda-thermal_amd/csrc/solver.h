@@ -93,6 +93,7 @@ struct SolverBase {
     virtual int set_x_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes,
                              int* used, double* rp, double* rl) = 0;
     virtual int nz() const = 0;
+    virtual int column_types() const = 0;
     virtual int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
                          int* hot_cell) = 0;
 };
@@ -108,6 +109,14 @@ struct Solver final : SolverBase {
     double *P = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
     double *z = nullptr, *invd = nullptr, *xnat = nullptr;
     T* pm = nullptr;  // multigrid search direction, in the V-cycle's precision (p itself in FP64)
+    // Column types of the fine level (levels.h: column_types); 0 types: the
+    // fine level reads its per-cell arrays.
+    int ntypes = 0;
+    bool allow_types = true;  // the batched engine, with kernels of its own, turns them off
+    unsigned short* d_ty = nullptr;
+    T* d_tab = nullptr;
+    unsigned char* d_tfl = nullptr;
+    Typed<T> typed() const { return Typed<T>{d_ty, d_tab, g0.nz}; }
     double* part = nullptr;
     State* st = nullptr;
     State* h_st = nullptr;  // pinned mirror
@@ -219,6 +228,24 @@ struct Solver final : SolverBase {
                 (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
                 return rc;
         }
+        // Column types of the fine level, where it runs the column walk.
+        if (allow_types && mg() && (long)g0.hx * g0.ny > kPPCols) {
+            ColumnTypes ct = th::column_types(m.nx, m.ny, m.nz, m.gE, m.gN, m.gU, m.gB, m.cap, kMaxColumnTypes);
+            if (ct.ntypes > 0) {
+                std::vector<unsigned short> slot((size_t)g0.plane, 0);  // padding slots: type 0, never read
+                for (int j = 0; j < m.ny; ++j)
+                    for (int i = 0; i < m.nx; ++i) slot[g0.pos(i, j)] = ct.type[(size_t)j * m.nx + i];
+                const std::vector<T> tab(ct.tab.begin(), ct.tab.end());
+                if ((rc = dalloc(&d_ty, slot.size())) || (rc = dalloc(&d_tab, tab.size())) ||
+                    (rc = dalloc(&d_tfl, ct.flags.size())))
+                    return rc;
+                HIPTRY(hipMemcpyAsync(d_ty, slot.data(), slot.size() * sizeof(unsigned short), hipMemcpyHostToDevice, s));
+                HIPTRY(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice, s));
+                HIPTRY(hipMemcpyAsync(d_tfl, ct.flags.data(), ct.flags.size(), hipMemcpyHostToDevice, s));
+                HIPTRY(hipStreamSynchronize(s));
+                ntypes = ct.ntypes;
+            }
+        }
         // The lowest levels run in one workgroup. FP32 with the lateral
         // restriction and nz <= 16: as many levels as fit in LDS (k_tail_lds).
         // Otherwise: levels with at most kTailCols columns (k_tail, L2).
@@ -288,6 +315,10 @@ struct Solver final : SolverBase {
         if (mg()) {
             for (DevLevel<T>& d : L) {
                 k_factor<T><<<col_grid(2 * d.g.hx, d.g.ny), col_block(), 0, s>>>(d.g, d.ops(), d.finv, st);
+                HIPTRY(hipGetLastError());
+            }
+            if (ntypes) {
+                k_factor_types<T><<<(ntypes + 63) / 64, 64, 0, s>>>(d_tab, d_tfl, ntypes, g0.nz, g0.nz, st);
                 HIPTRY(hipGetLastError());
             }
         } else {
@@ -382,6 +413,22 @@ struct Solver final : SolverBase {
             return TH_OK;
         }
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
+        if (k == 0 && ntypes) {
+#define TH_SWEEP(NZ)                                                                                            \
+    k_sweep_typed<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, typed(), d.b, d.x, colour, c.g, c.x, st, cs.part, \
+                                                            cs.nblk, cs.mode)
+            if (d.g.nz <= 4) TH_SWEEP(4);
+            else if (d.g.nz <= 8) TH_SWEEP(8);
+            else if (d.g.nz <= 12) TH_SWEEP(12);
+            else if (d.g.nz <= 16) TH_SWEEP(16);
+            else if (d.g.nz <= 24) TH_SWEEP(24);
+            else if (d.g.nz <= 32) TH_SWEEP(32);
+            else if (d.g.nz <= 48) TH_SWEEP(48);
+            else TH_SWEEP(64);
+#undef TH_SWEEP
+            HIPTRY(hipGetLastError());
+            return TH_OK;
+        }
 #define TH_SWEEP(NZ)                                                                                         \
     k_sweep<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, d.b, d.x, colour, c.g, c.x, st, \
                                                       cs.part, cs.nblk, cs.mode)
@@ -436,7 +483,9 @@ struct Solver final : SolverBase {
         const DevLevel<T>& c = L[k + 1];
         dim3 grid = col_grid(c.g.nx, c.g.ny);
         grid.z = c.g.nz;
-        if (fast)
+        if (fast && k == 0 && ntypes)
+            k_restrict_fast_typed<T><<<grid, col_block(), 0, s>>>(f.g, typed(), f.x, c.g, c.b, st);
+        else if (fast)
             k_restrict_fast<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.x, c.g, c.b, st);
         else
             k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
@@ -465,6 +514,20 @@ struct Solver final : SolverBase {
     int launch_apply_kernel(const PT* v) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const Ops<T> o = L[0].ops();
+        if (ntypes) {
+#define TH_APPLY(NZ) k_apply_typed<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, typed(), v, Ap, st, part_apply)
+            if (g0.nz <= 4) TH_APPLY(4);
+            else if (g0.nz <= 8) TH_APPLY(8);
+            else if (g0.nz <= 12) TH_APPLY(12);
+            else if (g0.nz <= 16) TH_APPLY(16);
+            else if (g0.nz <= 24) TH_APPLY(24);
+            else if (g0.nz <= 32) TH_APPLY(32);
+            else if (g0.nz <= 48) TH_APPLY(48);
+            else TH_APPLY(64);
+#undef TH_APPLY
+            HIPTRY(hipGetLastError());
+            return TH_OK;
+        }
 #define TH_APPLY(NZ) k_apply<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, o, v, Ap, st, part_apply)
         if (g0.nz <= 4) TH_APPLY(4);
         else if (g0.nz <= 8) TH_APPLY(8);
@@ -760,6 +823,7 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
     int nz() const override { return g0.nz; }
+    int column_types() const override { return ntypes; }
 
     int true_relres(double* out) override { return relres_of(x, out); }
 
@@ -874,10 +938,16 @@ struct Solver final : SolverBase {
             }
         };
         switch (which) {
-            case TH_K_SWEEP: *bytes = 4.5 * sz * N; break;                  // b, x(other), gE, gN, gU, 1/pivot; x
-            case TH_K_APPLY: *bytes = (8.0 + (mg() ? sz : 8.0) + 3.0 * sz) * N; break;  // Ap, p; gE, gN, gU
-            case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN; residual form: b, x, gE, gN, gU; coarse b
-                *bytes = (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
+            case TH_K_SWEEP:  // b, x(other), gE, gN, gU, 1/pivot; x. Typed: b, x(other), x, 2-byte types
+                *bytes = ntypes ? 1.5 * sz * N + 1.0 * g0.nx * g0.ny : 4.5 * sz * N;
+                break;
+            case TH_K_APPLY:  // Ap, p; gE, gN, gU (typed: 2-byte types)
+                *bytes = (8.0 + (mg() ? sz : 8.0)) * N + (ntypes ? 2.0 * g0.nx * g0.ny : 3.0 * sz * N);
+                break;
+            case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN (typed: 2-byte types of the red
+                                 // columns); residual form: b, x, gE, gN, gU; coarse b
+                *bytes = fast && ntypes ? 0.75 * sz * N + 1.0 * g0.nx * g0.ny
+                                        : (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
                 break;
             case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
             default: *bytes = 0.0;

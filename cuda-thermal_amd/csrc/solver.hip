@@ -117,6 +117,7 @@ int th_solver_heat_out(th_solver* s, double* out) {
     return s->impl->energy(nullptr, out, nullptr);
 }
 int th_solver_nlevels(th_solver* s) { return s ? s->impl->nlevels() : -1; }
+int th_solver_column_types(th_solver* s) { return s ? s->impl->column_types() : -1; }
 int th_solver_colorize(th_solver* s, int l0, int l1, float rise_hi, unsigned char* rgba, float* hot_rise,
                        int* hot_cell) {
     TH_S(s);

@@ -123,6 +123,7 @@ SIGNATURES = {
     "th_solver_true_relres": (C.c_int, [_vp, _dp]),
     "th_solver_heat_out": (C.c_int, [_vp, _dp]),
     "th_solver_nlevels": (C.c_int, [_vp]),
+    "th_solver_column_types": (C.c_int, [_vp]),
     "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_precondition": (C.c_int, [_vp, _dp, _dp]),
@@ -421,6 +422,10 @@ class Solver:
 
     def nlevels(self):
         return lib().th_solver_nlevels(self.h)
+
+    def column_types(self):
+        """Distinct fine-level column types in use (0: per-cell arrays)."""
+        return lib().th_solver_column_types(self.h)
 
     def colorize(self, l0, l1, rise_hi, image=True):
         """Die map of the current T (viewer.cu:68-87): the hottest rise of

@@ -162,6 +162,11 @@ TH_API int th_solver_true_relres(th_solver* s, double* out);
 /* heat_out (stack3d.h:254) at the current x, on the device. */
 TH_API int th_solver_heat_out(th_solver* s, double* out);
 TH_API int th_solver_nlevels(th_solver* s);
+/* Number of distinct fine-level column types the solver found (no reference
+ * counterpart; introspection): > 0 when the fine level's kernels read a
+ * 2-byte type per column and a table of the types' faces instead of the
+ * per-cell arrays, 0 when they read the arrays, -1 for a null solver. */
+TH_API int th_solver_column_types(th_solver* s);
 /* Die map of the current x (the viewer's k_colorize, viewer.cu:68-87): for
  * each column (i, j) the hottest temperature rise v over planes l0..l1 (at
  * least 0), coloured on [0, rise_hi] with the viewer's five-stop palette

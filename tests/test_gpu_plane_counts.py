@@ -104,6 +104,7 @@ def test_vcycle_bitwise_large_levels(nz, precision, mg_double, n):
     m.add_tsv_region(active[0], 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
     m.finalize()
     s = T.Solver(m, precision=precision)
+    assert s.column_types() > 0  # the fine level reads column types
     o = O.Solver(m.arrays(), "mg", mg_double=mg_double, fast_restrict=True)
     for dt in (0.0, 1e-3):
         s.set_dt(dt)
@@ -126,3 +127,23 @@ def test_steady_large_levels_vs_oracle(precision):
     assert st["converged"] and abs(st["iters"] - ref["iters"]) <= 1, (st, ref["iters"], O.true_relres(a, x))
     assert np.abs(x - ref["T"]).max() < 1e-8 * ref["T"].max()
     assert O.true_relres(a, x) <= 1.5e-10
+
+
+@pytest.mark.parametrize("nz", [1, 6, 16])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("dt", [0.0, 1e-3])
+def test_operator_bitwise_large_levels(nz, precision, dt):
+    """The fine-level apply on a grid large enough for column types (the
+    faces read from a per-type table): bit-identical to the oracle's."""
+    layers = STACKS[nz] if nz != 16 else [SI + (4, 1), BOND + (2, 0), SI + (4, 1), TIM + (2, 0), CU + (4, 0)]
+    m = T.Model.from_layers(layers, 301, 171)
+    m.add_uniform_power(0, 20.0)
+    m.add_tsv_region(0, 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
+    m.finalize()
+    s = T.Solver(m, precision=precision)
+    assert s.column_types() > 0
+    s.set_dt(dt)
+    o = O.Solver(m.arrays(), "mg")
+    o.set_dt(dt)
+    x = np.random.default_rng(nz).standard_normal(m.n)
+    assert np.array_equal(s.apply(x), o.apply(x))
