@@ -66,7 +66,14 @@ struct DevLevel {
     Geo g;
     T *gE = nullptr, *gN = nullptr, *gU = nullptr, *gB = nullptr, *cap = nullptr;
     T *finv = nullptr, *x = nullptr, *b = nullptr;
+    // column types (levels.h: column_types) of a level run by the column
+    // walk; 0 types: the level's kernels read the per-cell arrays
+    int ntypes = 0;
+    unsigned short* ty = nullptr;
+    T* tab = nullptr;
+    unsigned char* tfl = nullptr;
     Ops<T> ops() const { return {gE, gN, gU, gB, cap}; }
+    Typed<T> typed() const { return Typed<T>{ty, tab, g.nz}; }
 };
 
 struct SolverBase {
@@ -109,14 +116,7 @@ struct Solver final : SolverBase {
     double *P = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
     double *z = nullptr, *invd = nullptr, *xnat = nullptr;
     T* pm = nullptr;  // multigrid search direction, in the V-cycle's precision (p itself in FP64)
-    // Column types of the fine level (levels.h: column_types); 0 types: the
-    // fine level reads its per-cell arrays.
-    int ntypes = 0;
-    bool allow_types = true;  // the batched engine, with kernels of its own, turns them off
-    unsigned short* d_ty = nullptr;
-    T* d_tab = nullptr;
-    unsigned char* d_tfl = nullptr;
-    Typed<T> typed() const { return Typed<T>{d_ty, d_tab, g0.nz}; }
+    bool allow_types = true;  // column types; the batched engine, with kernels of its own, turns them off
     double* part = nullptr;
     State* st = nullptr;
     State* h_st = nullptr;  // pinned mirror
@@ -215,6 +215,26 @@ struct Solver final : SolverBase {
             T* dst[5] = {d.gE, d.gN, d.gU, d.cap, d.gB};
             for (int a = 0; a < 5; ++a)
                 if ((rc = upload_split<float, T>(d.g, a == 4 ? 1 : d.g.nz, src[a], dst[a]))) return rc;
+            // column types where the level runs the column walk
+            if (allow_types && mg() && (long)d.g.hx * d.g.ny > kPPCols) {
+                const ColumnTypes ct = th::column_types(d.g.nx, d.g.ny, d.g.nz, src[0], src[1], src[2], src[4],
+                                                        src[3], kMaxColumnTypes);
+                if (ct.ntypes > 0) {
+                    std::vector<unsigned short> slot((size_t)d.g.plane, 0);  // padding slots: type 0, never read
+                    for (int j = 0; j < d.g.ny; ++j)
+                        for (int i = 0; i < d.g.nx; ++i) slot[d.g.pos(i, j)] = ct.type[(size_t)j * d.g.nx + i];
+                    const std::vector<T> tab(ct.tab.begin(), ct.tab.end());
+                    if ((rc = dalloc(&d.ty, slot.size())) || (rc = dalloc(&d.tab, tab.size())) ||
+                        (rc = dalloc(&d.tfl, ct.flags.size())))
+                        return rc;
+                    HIPTRY(hipMemcpyAsync(d.ty, slot.data(), slot.size() * sizeof(unsigned short),
+                                          hipMemcpyHostToDevice, s));
+                    HIPTRY(hipMemcpyAsync(d.tab, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice, s));
+                    HIPTRY(hipMemcpyAsync(d.tfl, ct.flags.data(), ct.flags.size(), hipMemcpyHostToDevice, s));
+                    HIPTRY(hipStreamSynchronize(s));
+                    d.ntypes = ct.ntypes;
+                }
+            }
             const dim3 cg = col_grid(2 * d.g.hx, d.g.ny);
             maxblk = std::max(maxblk, (long)cg.x * cg.y);
             if (!mg()) break;
@@ -227,24 +247,6 @@ struct Solver final : SolverBase {
             if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_stencil)) ||
                 (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
                 return rc;
-        }
-        // Column types of the fine level, where it runs the column walk.
-        if (allow_types && mg() && (long)g0.hx * g0.ny > kPPCols) {
-            ColumnTypes ct = th::column_types(m.nx, m.ny, m.nz, m.gE, m.gN, m.gU, m.gB, m.cap, kMaxColumnTypes);
-            if (ct.ntypes > 0) {
-                std::vector<unsigned short> slot((size_t)g0.plane, 0);  // padding slots: type 0, never read
-                for (int j = 0; j < m.ny; ++j)
-                    for (int i = 0; i < m.nx; ++i) slot[g0.pos(i, j)] = ct.type[(size_t)j * m.nx + i];
-                const std::vector<T> tab(ct.tab.begin(), ct.tab.end());
-                if ((rc = dalloc(&d_ty, slot.size())) || (rc = dalloc(&d_tab, tab.size())) ||
-                    (rc = dalloc(&d_tfl, ct.flags.size())))
-                    return rc;
-                HIPTRY(hipMemcpyAsync(d_ty, slot.data(), slot.size() * sizeof(unsigned short), hipMemcpyHostToDevice, s));
-                HIPTRY(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice, s));
-                HIPTRY(hipMemcpyAsync(d_tfl, ct.flags.data(), ct.flags.size(), hipMemcpyHostToDevice, s));
-                HIPTRY(hipStreamSynchronize(s));
-                ntypes = ct.ntypes;
-            }
         }
         // The lowest levels run in one workgroup. FP32 with the lateral
         // restriction and nz <= 16: as many levels as fit in LDS (k_tail_lds).
@@ -317,10 +319,11 @@ struct Solver final : SolverBase {
                 k_factor<T><<<col_grid(2 * d.g.hx, d.g.ny), col_block(), 0, s>>>(d.g, d.ops(), d.finv, st);
                 HIPTRY(hipGetLastError());
             }
-            if (ntypes) {
-                k_factor_types<T><<<(ntypes + 63) / 64, 64, 0, s>>>(d_tab, d_tfl, ntypes, g0.nz, g0.nz, st);
-                HIPTRY(hipGetLastError());
-            }
+            for (DevLevel<T>& d : L)
+                if (d.ntypes) {
+                    k_factor_types<T><<<(d.ntypes + 63) / 64, 64, 0, s>>>(d.tab, d.tfl, d.ntypes, d.g.nz, d.g.nz, st);
+                    HIPTRY(hipGetLastError());
+                }
         } else {
             const DevLevel<T>& d = L[0];
             k_invdiag<T><<<col_grid(2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, d.ops(), invd, st);
@@ -413,9 +416,9 @@ struct Solver final : SolverBase {
             return TH_OK;
         }
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
-        if (k == 0 && ntypes) {
+        if (d.ntypes) {
 #define TH_SWEEP(NZ)                                                                                            \
-    k_sweep_typed<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, typed(), d.b, d.x, colour, c.g, c.x, st, cs.part, \
+    k_sweep_typed<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st, cs.part, \
                                                             cs.nblk, cs.mode)
             if (d.g.nz <= 4) TH_SWEEP(4);
             else if (d.g.nz <= 8) TH_SWEEP(8);
@@ -483,8 +486,8 @@ struct Solver final : SolverBase {
         const DevLevel<T>& c = L[k + 1];
         dim3 grid = col_grid(c.g.nx, c.g.ny);
         grid.z = c.g.nz;
-        if (fast && k == 0 && ntypes)
-            k_restrict_fast_typed<T><<<grid, col_block(), 0, s>>>(f.g, typed(), f.x, c.g, c.b, st);
+        if (fast && f.ntypes)
+            k_restrict_fast_typed<T><<<grid, col_block(), 0, s>>>(f.g, f.typed(), f.x, c.g, c.b, st);
         else if (fast)
             k_restrict_fast<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.x, c.g, c.b, st);
         else
@@ -514,8 +517,8 @@ struct Solver final : SolverBase {
     int launch_apply_kernel(const PT* v) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const Ops<T> o = L[0].ops();
-        if (ntypes) {
-#define TH_APPLY(NZ) k_apply_typed<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, typed(), v, Ap, st, part_apply)
+        if (L[0].ntypes) {
+#define TH_APPLY(NZ) k_apply_typed<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, L[0].typed(), v, Ap, st, part_apply)
             if (g0.nz <= 4) TH_APPLY(4);
             else if (g0.nz <= 8) TH_APPLY(8);
             else if (g0.nz <= 12) TH_APPLY(12);
@@ -823,7 +826,7 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
     int nz() const override { return g0.nz; }
-    int column_types() const override { return ntypes; }
+    int column_types() const override { return L.empty() ? 0 : L[0].ntypes; }
 
     int true_relres(double* out) override { return relres_of(x, out); }
 
@@ -939,14 +942,14 @@ struct Solver final : SolverBase {
         };
         switch (which) {
             case TH_K_SWEEP:  // b, x(other), gE, gN, gU, 1/pivot; x. Typed: b, x(other), x, 2-byte types
-                *bytes = ntypes ? 1.5 * sz * N + 1.0 * g0.nx * g0.ny : 4.5 * sz * N;
+                *bytes = L[0].ntypes ? 1.5 * sz * N + 1.0 * g0.nx * g0.ny : 4.5 * sz * N;
                 break;
             case TH_K_APPLY:  // Ap, p; gE, gN, gU (typed: 2-byte types)
-                *bytes = (8.0 + (mg() ? sz : 8.0)) * N + (ntypes ? 2.0 * g0.nx * g0.ny : 3.0 * sz * N);
+                *bytes = (8.0 + (mg() ? sz : 8.0)) * N + (L[0].ntypes ? 2.0 * g0.nx * g0.ny : 3.0 * sz * N);
                 break;
             case TH_K_RESTRICT:  // lateral: x (one colour), gE, gN (typed: 2-byte types of the red
                                  // columns); residual form: b, x, gE, gN, gU; coarse b
-                *bytes = fast && ntypes ? 0.75 * sz * N + 1.0 * g0.nx * g0.ny
+                *bytes = fast && L[0].ntypes ? 0.75 * sz * N + 1.0 * g0.nx * g0.ny
                                         : (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
                 break;
             case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
