@@ -936,6 +936,19 @@ struct Solver final : SolverBase {
                     return TH_OK;
                 case TH_K_VCYCLE:
                     return vcycle(0);
+                case TH_K_UPDATE: {
+                    const unsigned vb = vec_blocks(n0);
+                    if (opt.beta == TH_BETA_FLEXIBLE)
+                        k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x,
+                                                                           nullptr, nullptr, st, part_upd, part_apply,
+                                                                           (int)nblk_stencil);
+                    else
+                        k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr,
+                                                                      nullptr, st, part_upd, part_apply,
+                                                                      (int)nblk_stencil);
+                    HIPTRY(hipGetLastError());
+                    return TH_OK;
+                }
                 default:
                     return fail(TH_ERR_ARG, "unknown kernel id");
             }
@@ -953,11 +966,15 @@ struct Solver final : SolverBase {
                                         : (fast ? 2.5 : 5.0) * sz * N + 0.25 * sz * N;
                 break;
             case TH_K_COPY: *bytes = 16.0 * (double)n0; break;
+            case TH_K_UPDATE:  // r, Ap, z_old (flexible beta); r, b0
+                *bytes = (24.0 + (opt.beta == TH_BETA_FLEXIBLE ? 2.0 : 1.0) * sz) * N;
+                break;
             default: *bytes = 0.0;
         }
         k_reopen<<<1, 1, 0, s>>>(st);  // kernels return at once on a finished solve
         HIPTRY(hipGetLastError());
         int rc;
+        if (which == TH_K_UPDATE && (rc = launch_apply_kernel(pm))) return rc;  // the partials alpha is formed from
         if ((rc = launch())) return rc;
         hipEvent_t e0, e1;
         HIPTRY(hipEventCreate(&e0));

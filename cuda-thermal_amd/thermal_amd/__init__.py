@@ -445,7 +445,7 @@ class Solver:
         _check(lib().th_solver_precondition(self.h, _d(r), _d(z)))
         return z
 
-    KERNELS = {"sweep": 0, "apply": 1, "restrict": 2, "copy": 3, "vcycle": 4}
+    KERNELS = {"sweep": 0, "apply": 1, "restrict": 2, "copy": 3, "vcycle": 4, "update": 5}
 
     def time_kernel(self, which, reps=20):
         """(average ms per launch, algorithmic bytes per launch) of one

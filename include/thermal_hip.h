@@ -196,8 +196,10 @@ TH_API int th_solver_apply_operator(th_solver* s, const double* x, double* y);
  *   TH_K_APPLY    fine-level FP64 operator apply Ap = (A + C/dt) p
  *   TH_K_RESTRICT fine-to-first-coarse restriction
  *   TH_K_COPY     streaming copy of one FP64 vector (bandwidth reference)
- *   TH_K_VCYCLE   one whole V-cycle (bytes reported as 0) */
-enum { TH_K_SWEEP = 0, TH_K_APPLY = 1, TH_K_RESTRICT = 2, TH_K_COPY = 3, TH_K_VCYCLE = 4 };
+ *   TH_K_VCYCLE   one whole V-cycle (bytes reported as 0)
+ *   TH_K_UPDATE   the iteration's residual update: r -= alpha Ap, the V-cycle's
+ *                 right-hand side, ||r||^2 and r.z_old (multigrid solvers) */
+enum { TH_K_SWEEP = 0, TH_K_APPLY = 1, TH_K_RESTRICT = 2, TH_K_COPY = 3, TH_K_VCYCLE = 4, TH_K_UPDATE = 5 };
 TH_API int th_solver_time_kernel(th_solver* s, int which, int reps, double* avg_ms, double* bytes);
 
 /* Steady solve from T = 0 with timing (gpu_pcg, pcg3d_gpu.cu:686-719). */
