@@ -123,6 +123,60 @@ struct Typed {
     __device__ __forceinline__ TypeRow<T> row(int t, int l) const {
         return reinterpret_cast<const TypeRow<T>*>(tab)[(long)t * nzp + l];
     }
+    // The same row for a wave-uniform t, read once per wave through the
+    // scalar cache (constant address space: the table does not change while
+    // a kernel runs) instead of once per lane through L1.
+    __device__ __forceinline__ TypeRow<T> urow(int t, int l) const {
+        typedef const __attribute__((address_space(4))) T* CPtr;
+        const CPtr c = (CPtr)tab + ((long)t * nzp + l) * kTypeFields;
+        TypeRow<T> r;
+#pragma unroll
+        for (int k = 0; k < kTypeFields; ++k) r.v[k] = c[k];
+        return r;
+    }
+    // The wave's representative type through urow, then the lanes whose
+    // column has another type load their own row.
+    __device__ __forceinline__ TypeRow<T> row(int t, int rep, int l) const {
+        TypeRow<T> r = urow(rep, l);
+        if (t != rep) r = row(t, l);
+        return r;
+    }
+};
+
+// A wave-uniform type from the active lanes: the first active lane of the
+// wave's upper half (else of the lower half), so that a domain-edge column,
+// which has a type of its own and sits in the first or last lane of a row,
+// is not the representative of its wave.
+__device__ __forceinline__ int wave_rep_type(int t) {
+    const unsigned long long m = __ballot(1);
+    const unsigned hi = (unsigned)(m >> 32);
+    const int lane = hi ? 32 + __builtin_ctz(hi) : __builtin_ctzll(m);
+    return __builtin_amdgcn_readlane(t, lane);
+}
+
+// Plane-count buckets of the fine-level column kernels: an instantiation for
+// NZ in {4, 8, 12, 16, 24, 32, 48, 64} runs the levels with
+// nz_floor(NZ) < nz <= NZ (the launch ladders pick the smallest bucket that
+// holds nz), so planes l <= nz_floor(NZ) exist in every level it runs and
+// need no run-time test.
+constexpr int nz_floor(int NZ) {
+    return NZ <= 4 ? 0 : NZ <= 8 ? 4 : NZ <= 12 ? 8 : NZ <= 16 ? 12 : NZ <= 24 ? 16 : NZ <= 32 ? 24 : NZ <= 48 ? 32 : 48;
+}
+template <int NZ>
+__device__ __forceinline__ bool plane_live(int l, int nz) {
+    return l <= nz_floor(NZ) || l < nz;
+}
+
+// Two-element vector of T (16-byte loads of double pairs, 8-byte of float).
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    typedef float2 type;
+};
+template <>
+struct Vec2<double> {
+    typedef double2 type;
 };
 
 // ---- reductions -------------------------------------------------------------
@@ -205,6 +259,7 @@ __device__ void block_sum_partials(const double* __restrict__ part, int nblk, do
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+#pragma unroll 4
     for (int b = tid; b < nblk; b += nthr)
 #pragma unroll
         for (int k = 0; k < NV; ++k) acc[k] += part[(size_t)b * NV + k];

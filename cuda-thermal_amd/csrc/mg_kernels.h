@@ -36,6 +36,25 @@ __device__ __forceinline__ Parents parents(const Geo& gc, const Col& c, int i, i
                    gc.pos(i >> 1, (c.hs ? j - 1 : j) >> 1), gc.pos(i >> 1, (c.hn ? j + 1 : j) >> 1)};
 }
 
+// The same four parents in three loads: one of the W/E neighbours and one of
+// the S/N neighbours always share the column's own parent p0; px and py are
+// the other two, and wx/ex/sy/nn say which neighbour's parent is px or py.
+struct Parents3 {
+    int p0, px, py;
+    bool wx, ex, sy, nn;
+};
+__device__ __forceinline__ Parents3 parents3(const Geo& gc, const Col& c, int i, int j) {
+    Parents3 q;
+    q.wx = c.hw && !(i & 1);
+    q.ex = c.he && (i & 1);
+    q.sy = c.hs && !(j & 1);
+    q.nn = c.hn && (j & 1);
+    q.p0 = gc.pos(i >> 1, j >> 1);
+    q.px = gc.pos(q.wx ? (i - 1) >> 1 : q.ex ? (i + 1) >> 1 : i >> 1, j >> 1);
+    q.py = gc.pos(i >> 1, q.sy ? (j - 1) >> 1 : q.nn ? (j + 1) >> 1 : j >> 1);
+    return q;
+}
+
 // Right-hand side of plane l of column c with its lateral neighbours held
 // fixed: b + sum G x_nb (SW_PROLONG: each neighbour plus its coarse parent's
 // correction; SW_ZERO: neighbours taken as 0).
@@ -352,54 +371,67 @@ __global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const 
     }
 }
 
-// sweep_column with the faces from the column's type.
-template <class T, int NZ, int MODE>
+// sweep_column with the faces from the column's type. UNI: every column of
+// the wave has type t (a wave-uniform value), so the rows come through the
+// scalar cache. Planes past nz (NZ is a size bucket) re-read the top plane,
+// so every load is unconditional and in bounds and the loads of all planes
+// can be in flight together; those planes' values are never used. C3
+// (SW_PROLONG): the coarse parents in three loads per plane (parents3).
+template <class T, int NZ, int MODE, bool UNI, bool C3>
 __device__ __forceinline__ void sweep_column_typed(const Geo& g, const Typed<T>& y, const T* __restrict__ b,
-                                                   T* __restrict__ x, int i, int j, const Geo& gc,
-                                                   const T* __restrict__ xc) {
+                                                   T* __restrict__ x, const Col& c, int t, const Geo& gc,
+                                                   const T* __restrict__ xc, const Parents& pa,
+                                                   const Parents3& p3) {
     const int nz = g.nz;
-    const Col c = make_col(g, i, j);
-    const Parents pa = MODE == SW_PROLONG ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
-    const int t = y.ty[c.c0];
     T s[NZ], inv[NZ], gu[NZ];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) {
-        if (l < nz) {
-            const long base = (long)l * g.plane, id = base + c.c0;
-            const TypeRow<T> r = y.row(t, l);
-            T v = b[id];
-            if (MODE != SW_ZERO) {  // column_rhs
-                T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
-                if (MODE == SW_PROLONG) {
-                    const long cb = (long)l * gc.plane;
-                    xw = xw + xc[cb + pa.w];
-                    xe = xe + xc[cb + pa.e];
-                    xsv = xsv + xc[cb + pa.s];
-                    xn = xn + xc[cb + pa.n];
-                }
-                const T Gw = r.v[TF_W], Ge = r.v[TF_E], Gs = r.v[TF_S], Gn = r.v[TF_N];
-                T ls = 0;
-                if (c.hw) ls += Gw * xw;
-                if (c.he) ls += Ge * xe;
-                if (c.hs) ls += Gs * xsv;
-                if (c.hn) ls += Gn * xn;
-                v = v + ls;
+        const int lc = l < nz ? l : nz - 1;
+        const long base = (long)lc * g.plane, id = base + c.c0;
+        const TypeRow<T> r = UNI ? y.urow(t, lc) : y.row(t, lc);
+        T v = b[id];
+        if (MODE != SW_ZERO) {  // column_rhs
+            T xw = x[base + c.pw], xe = x[base + c.pe], xsv = x[base + c.ps], xn = x[base + c.pn];
+            if (MODE == SW_PROLONG && C3) {
+                const long cb = (long)lc * gc.plane;
+                const T c0 = xc[cb + p3.p0], cx = xc[cb + p3.px], cy = xc[cb + p3.py];
+                xw = xw + (p3.wx ? cx : c0);
+                xe = xe + (p3.ex ? cx : c0);
+                xsv = xsv + (p3.sy ? cy : c0);
+                xn = xn + (p3.nn ? cy : c0);
+            } else if (MODE == SW_PROLONG) {
+                const long cb = (long)lc * gc.plane;
+                xw = xw + xc[cb + pa.w];
+                xe = xe + xc[cb + pa.e];
+                xsv = xsv + xc[cb + pa.s];
+                xn = xn + xc[cb + pa.n];
             }
-            s[l] = v;
-            inv[l] = r.v[TF_FI];
-            gu[l] = r.v[TF_U];
+            // Absent faces add (+0) * x: ls starts at +0 and never becomes -0,
+            // so the sum is the one with those terms skipped, without the
+            // branches that would hold a load back until its use.
+            const T Gw = c.hw ? r.v[TF_W] : T(0), Ge = c.he ? r.v[TF_E] : T(0);
+            const T Gs = c.hs ? r.v[TF_S] : T(0), Gn = c.hn ? r.v[TF_N] : T(0);
+            T ls = 0;
+            ls += Gw * xw;
+            ls += Ge * xe;
+            ls += Gs * xsv;
+            ls += Gn * xn;
+            v = v + ls;
         }
+        s[l] = v;
+        inv[l] = r.v[TF_FI];
+        gu[l] = r.v[TF_U];
     }
     column_thomas<T, NZ>(nz, s, inv, gu);
 #pragma unroll
     for (int l = 0; l < NZ; ++l)
         if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
 }
-template <class T, int NZ, int MODE>
-__global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x,
-                                                     int colour, Geo gc, const T* __restrict__ xc, State* st,
-                                                     const double* __restrict__ cons_part, int cons_nblk,
-                                                     int cons_mode) {
+template <class T, int NZ, int MODE, bool C3>
+__device__ __forceinline__ void sweep_typed_body(const Geo& g, const Typed<T>& y, const T* __restrict__ b,
+                                                 T* __restrict__ x, int colour, const Geo& gc,
+                                                 const T* __restrict__ xc, State* st,
+                                                 const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
     const int t = blockIdx.x * kColsX + threadIdx.x;
@@ -407,7 +439,33 @@ __global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T*
     if (j >= g.ny) return;
     const int i = 2 * t + ((j + colour) & 1);
     if (i >= g.nx) return;
-    sweep_column_typed<T, NZ, MODE>(g, y, b, x, i, j, gc, xc);
+    const Col c = make_col(g, i, j);
+    const Parents pa = MODE == SW_PROLONG && !C3 ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
+    const Parents3 p3 = MODE == SW_PROLONG && C3 ? parents3(gc, c, i, j) : Parents3{0, 0, 0, false, false, false, false};
+    const int ty = y.ty[c.c0];
+    const int rep = wave_rep_type(ty);
+    if (__all(ty == rep))
+        sweep_column_typed<T, NZ, MODE, true, C3>(g, y, b, x, c, rep, gc, xc, pa, p3);
+    else
+        sweep_column_typed<T, NZ, MODE, false, C3>(g, y, b, x, c, ty, gc, xc, pa, p3);
+}
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x,
+                                                     int colour, Geo gc, const T* __restrict__ xc, State* st,
+                                                     const double* __restrict__ cons_part, int cons_nblk,
+                                                     int cons_mode) {
+    sweep_typed_body<T, NZ, MODE, false>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+}
+// The FP32 prolongation sweep: three coarse loads per plane and at least 4
+// waves per SIMD. With every plane's loads unconditional the compiler would
+// otherwise hold them all at 160 VGPRs; at 1024^2 x 16 with the caches
+// flushed this runs in 52 us against 62 for the per-plane form and 75 at the
+// compiler's own register choice. (FP64 runs fastest at its own choice.)
+template <class T, int NZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_typed_prolong3(
+    Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc, const T* __restrict__ xc,
+    State* st, const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
+    sweep_typed_body<T, NZ, SW_PROLONG, true>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 // restrict_cell_fast with the red children's faces from their types.
@@ -425,14 +483,16 @@ __global__ __launch_bounds__(256) void k_restrict_fast_typed(Geo f, Typed<T> y, 
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const Col c = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
-        const TypeRow<T> w = y.row(y.ty[c.c0], l);
-        const T Gw = w.v[TF_W], Ge = w.v[TF_E], Gs = w.v[TF_S], Gn = w.v[TF_N];
+        const int t = y.ty[c.c0];
+        const TypeRow<T> w = y.row(t, wave_rep_type(t), l);
+        const T Gw = c.hw ? w.v[TF_W] : T(0), Ge = c.he ? w.v[TF_E] : T(0);
+        const T Gs = c.hs ? w.v[TF_S] : T(0), Gn = c.hn ? w.v[TF_N] : T(0);
         const T xw = fx[base + c.pw], xe = fx[base + c.pe], xs = fx[base + c.ps], xn = fx[base + c.pn];
-        T s = 0;
-        if (c.hw) s += Gw * xw;
-        if (c.he) s += Ge * xe;
-        if (c.hs) s += Gs * xs;
-        if (c.hn) s += Gn * xn;
+        T s = 0;  // absent faces add (+0) * x, as in sweep_column_typed
+        s += Gw * xw;
+        s += Ge * xe;
+        s += Gs * xs;
+        s += Gn * xn;
         r[k] = s;
     }
     T sum = 0;

@@ -138,53 +138,63 @@ __device__ __forceinline__ void apply_body(const Blk& B, const Geo& g, const Ops
     }
     block_partials<1>(v, part, B.flat());
 }
-// apply_column with the faces from the column's type (fine level).
-template <class T, int NZ, class PT, bool TR>
+// apply_column with the faces from the column's type (fine level). UNI: the
+// wave's columns all have type t, read through the scalar cache. Planes past
+// nz (NZ is a size bucket) and the plane above the top one re-read planes
+// that exist, so every load is unconditional and the loads of all planes can
+// be in flight together; the values they give are never used.
+template <class T, int NZ, class PT, bool TR, bool UNI>
 __device__ __forceinline__ double apply_column_typed(const Geo& g, const Typed<T>& y, const PT* __restrict__ p,
-                                                     double* __restrict__ Ap, const Col& c, double inv_dt) {
+                                                     double* __restrict__ Ap, const Col& c, int t, double inv_dt) {
     const int nz = g.nz;
-    const int t = y.ty[c.c0];
     double v = 0.0;
-    double pb = 0.0, pc = (double)p[c.c0], gdn = (double)y.row(t, 0).v[TF_B];
+    double pb = 0.0, pc = (double)p[c.c0];
+    double gdn = (double)(UNI ? y.urow(t, 0) : y.row(t, 0)).v[TF_B];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) {
-        if (l < nz) {
-            const long base = (long)l * g.plane, id = base + c.c0;
-            const TypeRow<T> r = y.row(t, l);
-            const double pu = (l < nz - 1) ? (double)p[id + g.plane] : 0.0;
-            const double gu = (double)r.v[TF_U];
-            const double Gw = (double)r.v[TF_W], Ge = (double)r.v[TF_E];
-            const double Gs = (double)r.v[TF_S], Gn = (double)r.v[TF_N];
-            const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
-                         xn = (double)p[base + c.pn];
-            double d = 0.0, s = 0.0;
-            if (c.hw) d += Gw;
-            if (c.he) d += Ge;
-            if (c.hs) d += Gs;
-            if (c.hn) d += Gn;
-            d += gu;
-            d += gdn;
-            if (TR) d = d + (double)r.v[TF_C] * inv_dt;
-            if (c.hw) s += Gw * xw;
-            if (c.he) s += Ge * xe;
-            if (c.hs) s += Gs * xs;
-            if (c.hn) s += Gn * xn;
-            if (l > 0) s += gdn * pb;
-            if (l < nz - 1) s += gu * pu;
-            const double a = d * pc - s;
+        const int lc = l < nz ? l : nz - 1;
+        const bool up = l < nz - 1;
+        const long base = (long)lc * g.plane, id = base + c.c0;
+        const TypeRow<T> r = UNI ? y.urow(t, lc) : y.row(t, lc);
+        const double pu_ld = (double)p[id + (up ? g.plane : 0)];
+        const double pu = up ? pu_ld : 0.0;
+        const double gu = (double)r.v[TF_U];
+        // Absent faces contribute (+0) and (+0) * x: d and s start at +0 and
+        // never become -0, so both sums equal the ones with those terms
+        // skipped, and no load waits behind a branch.
+        const double Gw = c.hw ? (double)r.v[TF_W] : 0.0, Ge = c.he ? (double)r.v[TF_E] : 0.0;
+        const double Gs = c.hs ? (double)r.v[TF_S] : 0.0, Gn = c.hn ? (double)r.v[TF_N] : 0.0;
+        const double xw = (double)p[base + c.pw], xe = (double)p[base + c.pe], xs = (double)p[base + c.ps],
+                     xn = (double)p[base + c.pn];
+        double d = 0.0, s = 0.0;
+        d += Gw;
+        d += Ge;
+        d += Gs;
+        d += Gn;
+        d += gu;
+        d += gdn;
+        if (TR) d = d + (double)r.v[TF_C] * inv_dt;
+        s += Gw * xw;
+        s += Ge * xe;
+        s += Gs * xs;
+        s += Gn * xn;
+        if (l > 0) s += gdn * pb;
+        if (up) s += gu * pu;
+        const double a = d * pc - s;
+        if (plane_live<NZ>(l, nz)) {
             Ap[id] = a;
             v += pc * a;
-            pb = pc;
-            pc = pu;
-            gdn = gu;
         }
+        pb = pc;
+        pc = pu;
+        gdn = gu;
     }
     return v;
 }
 template <class T, int NZ, class PT>
-__global__ __launch_bounds__(1024) void k_apply_typed(Geo g, Typed<T> y, const PT* __restrict__ p,
-                                                      double* __restrict__ Ap, const State* __restrict__ st,
-                                                      double* __restrict__ part) {
+__device__ __forceinline__ void apply_typed_body(const Geo& g, const Typed<T>& y, const PT* __restrict__ p,
+                                                 double* __restrict__ Ap, const State* __restrict__ st,
+                                                 double* __restrict__ part) {
     if (st->done) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -193,10 +203,32 @@ __global__ __launch_bounds__(1024) void k_apply_typed(Geo g, Typed<T> y, const P
     int i;
     if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
         const Col c = make_col(g, i, j);
-        v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true>(g, y, p, Ap, c, inv_dt)
-                             : apply_column_typed<T, NZ, PT, false>(g, y, p, Ap, c, inv_dt);
+        const int ty = y.ty[c.c0];
+        const int rep = wave_rep_type(ty);
+        if (__all(ty == rep))
+            v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true, true>(g, y, p, Ap, c, rep, inv_dt)
+                                 : apply_column_typed<T, NZ, PT, false, true>(g, y, p, Ap, c, rep, inv_dt);
+        else
+            v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true, false>(g, y, p, Ap, c, ty, inv_dt)
+                                 : apply_column_typed<T, NZ, PT, false, false>(g, y, p, Ap, c, ty, inv_dt);
     }
     block_partials<1>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+}
+// 64 x 4 blocks. FP64 faces: the compiler's own register choice (106 us at
+// 1024^2 x 16 with the caches flushed, against 116 for 64 x 16 blocks).
+// FP32 faces (k_apply_typed_occ): at least 6 waves per SIMD, 62 us against
+// 72 us at the compiler's own choice.
+template <class T, int NZ, class PT>
+__global__ __launch_bounds__(256) void k_apply_typed(Geo g, Typed<T> y, const PT* __restrict__ p,
+                                                     double* __restrict__ Ap, const State* __restrict__ st,
+                                                     double* __restrict__ part) {
+    apply_typed_body<T, NZ, PT>(g, y, p, Ap, st, part);
+}
+template <class T, int NZ, class PT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_apply_typed_occ(
+    Geo g, Typed<T> y, const PT* __restrict__ p, double* __restrict__ Ap, const State* __restrict__ st,
+    double* __restrict__ part) {
+    apply_typed_body<T, NZ, PT>(g, y, p, Ap, st, part);
 }
 template <class T, int NZ, class PT = double>
 __global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const PT* __restrict__ p,
@@ -235,16 +267,47 @@ __device__ __forceinline__ void update_body(const Blk& B, long n, const double* 
     }
     double v[2] = {0.0, 0.0};
     const long stride = (long)B.gx * blockDim.x;
-    for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        if (MODE == UP_JACOBI) x[k] += alpha * p[k];
-        const double ri = r[k] - alpha * Ap[k];
-        r[k] = ri;
-        v[0] += ri * ri;
-        if (MODE == UP_JACOBI) {
+    const long k0 = (long)B.x * blockDim.x + threadIdx.x;
+    if (MODE == UP_JACOBI) {
+        for (long k = k0; k < n; k += stride) {
+            x[k] += alpha * p[k];
+            const double ri = r[k] - alpha * Ap[k];
+            r[k] = ri;
+            v[0] += ri * ri;
             const double zi = ri * invd[k];
             z[k] = zi;
             v[1] += ri * zi;
-        } else {
+        }
+    } else {
+        // Two cells per lane (16-byte loads of r and Ap): 98 against 105 us
+        // at 1024^2 x 16. A level's planes hold an even number of slots, so n
+        // is even; an odd last cell would go to lane 0 of block 0.
+        typedef typename Vec2<T>::type T2;
+        const long n2 = n >> 1;
+        for (long k = k0; k < n2; k += stride) {
+            const double2 a = reinterpret_cast<const double2*>(Ap)[k];
+            const double2 ro = reinterpret_cast<const double2*>(r)[k];
+            double2 ri;
+            ri.x = ro.x - alpha * a.x;
+            ri.y = ro.y - alpha * a.y;
+            reinterpret_cast<double2*>(r)[k] = ri;
+            v[0] += ri.x * ri.x;
+            v[0] += ri.y * ri.y;
+            if (MODE == UP_MG_FLEX) {
+                const T2 zo = reinterpret_cast<const T2*>(zold)[k];
+                v[1] += ri.x * (double)zo.x;
+                v[1] += ri.y * (double)zo.y;
+            }
+            T2 bv;
+            bv.x = (T)ri.x;
+            bv.y = (T)ri.y;
+            reinterpret_cast<T2*>(b0)[k] = bv;
+        }
+        if ((n & 1) && k0 == 0) {
+            const long k = n - 1;
+            const double ri = r[k] - alpha * Ap[k];
+            r[k] = ri;
+            v[0] += ri * ri;
             if (MODE == UP_MG_FLEX) v[1] += ri * (double)zold[k];
             b0[k] = (T)ri;
         }
@@ -308,8 +371,26 @@ __device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __re
     }
     const long stride = (long)B.gx * blockDim.x;
     if (!FIRST && x) {
+        // Two cells per lane, as in update_body (n even; an odd last cell
+        // goes to lane 0 of block 0).
+        typedef typename Vec2<PT>::type P2;
+        typedef typename Vec2<Z>::type Z2;
         const double alpha = st->alpha;
-        for (long k = (long)B.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const long k0 = (long)B.x * blockDim.x + threadIdx.x, n2 = n >> 1;
+        for (long k = k0; k < n2; k += stride) {
+            const P2 po = reinterpret_cast<const P2*>(p)[k];
+            const Z2 zz = reinterpret_cast<const Z2*>(z)[k];
+            double2 xx = reinterpret_cast<const double2*>(x)[k];
+            xx.x += alpha * (double)po.x;
+            xx.y += alpha * (double)po.y;
+            reinterpret_cast<double2*>(x)[k] = xx;
+            P2 pn;
+            pn.x = (PT)((double)zz.x + beta * (double)po.x);
+            pn.y = (PT)((double)zz.y + beta * (double)po.y);
+            reinterpret_cast<P2*>(p)[k] = pn;
+        }
+        if ((n & 1) && k0 == 0) {
+            const long k = n - 1;
             const double po = (double)p[k];
             x[k] += alpha * po;
             p[k] = (PT)((double)z[k] + beta * po);

@@ -242,9 +242,11 @@ struct Solver final : SolverBase {
         if ((rc = dalloc(&part, maxblk * 4))) return rc;
         {
             const dim3 sg = stencil_grid(2 * g0.hx, g0.ny);
+            const dim3 ag = L[0].ntypes ? col_grid(2 * g0.hx, g0.ny) : sg;  // launch_apply_kernel's grid
             nblk_stencil = sg.x * sg.y;
+            nblk_apply = ag.x * ag.y;
             nblk_vec = vec_blocks(n0);
-            if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_stencil)) ||
+            if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_apply)) ||
                 (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
                 return rc;
         }
@@ -379,7 +381,7 @@ struct Solver final : SolverBase {
     // Partial buffers, one per producing kernel, so a consumer can still read
     // its producer's partials while the next reduction runs.
     double *part_res = nullptr, *part_apply = nullptr, *part_upd = nullptr, *part_dot = nullptr;
-    unsigned nblk_stencil = 0, nblk_vec = 0;
+    unsigned nblk_stencil = 0, nblk_apply = 0, nblk_vec = 0;
 
     template <int NV, int EPI, unsigned KMAX = 0>
     int finalize(unsigned nblk, const double* pp = nullptr) {
@@ -417,9 +419,16 @@ struct Solver final : SolverBase {
         }
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
         if (d.ntypes) {
-#define TH_SWEEP(NZ)                                                                                            \
-    k_sweep_typed<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st, cs.part, \
-                                                            cs.nblk, cs.mode)
+            // FP32 prolongation: its own kernel (three coarse loads, occupancy floor)
+#define TH_SWEEP(NZ)                                                                                                  \
+    do {                                                                                                              \
+        if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value)                                            \
+            k_sweep_typed_prolong3<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st, \
+                                                                       cs.part, cs.nblk, cs.mode);                    \
+        else                                                                                                          \
+            k_sweep_typed<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st,    \
+                                                                    cs.part, cs.nblk, cs.mode);                       \
+    } while (0)
             if (d.g.nz <= 4) TH_SWEEP(4);
             else if (d.g.nz <= 8) TH_SWEEP(8);
             else if (d.g.nz <= 12) TH_SWEEP(12);
@@ -518,7 +527,15 @@ struct Solver final : SolverBase {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const Ops<T> o = L[0].ops();
         if (L[0].ntypes) {
-#define TH_APPLY(NZ) k_apply_typed<T, NZ, PT><<<grid, stencil_block(), 0, s>>>(g0, L[0].typed(), v, Ap, st, part_apply)
+            // 64 x 4 blocks (nblk_apply partials); FP32 faces with an occupancy floor
+#define TH_APPLY(NZ)                                                                                                  \
+    do {                                                                                                              \
+        const dim3 ag = col_grid(2 * g0.hx, g0.ny);                                                                   \
+        if constexpr (std::is_same<T, float>::value)                                                                  \
+            k_apply_typed_occ<T, NZ, PT><<<ag, col_block(), 0, s>>>(g0, L[0].typed(), v, Ap, st, part_apply);         \
+        else                                                                                                          \
+            k_apply_typed<T, NZ, PT><<<ag, col_block(), 0, s>>>(g0, L[0].typed(), v, Ap, st, part_apply);             \
+    } while (0)
             if (g0.nz <= 4) TH_APPLY(4);
             else if (g0.nz <= 8) TH_APPLY(8);
             else if (g0.nz <= 12) TH_APPLY(12);
@@ -547,7 +564,7 @@ struct Solver final : SolverBase {
     int launch_apply() {  // Ap = A p with alpha in the state (Jacobi path and tests)
         int rc;
         if ((rc = launch_apply_kernel(p))) return rc;
-        return finalize<1, FIN_ALPHA>(nblk_stencil, part_apply);
+        return finalize<1, FIN_ALPHA>(nblk_apply, part_apply);
     }
 
     // Start: residual, first preconditioner application, rz, p = z.
@@ -583,10 +600,10 @@ struct Solver final : SolverBase {
             if (opt.beta == TH_BETA_FLEXIBLE)
                 k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x, nullptr,
                                                                    nullptr, st, part_upd, part_apply,
-                                                                   (int)nblk_stencil);
+                                                                   (int)nblk_apply);
             else
                 k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr, nullptr,
-                                                              st, part_upd, part_apply, (int)nblk_stencil);
+                                                              st, part_upd, part_apply, (int)nblk_apply);
             HIPTRY(hipGetLastError());
             if ((rc = vcycle(0, Cons{part_upd, (int)vb, CONS_ITER}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
@@ -941,11 +958,11 @@ struct Solver final : SolverBase {
                     if (opt.beta == TH_BETA_FLEXIBLE)
                         k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x,
                                                                            nullptr, nullptr, st, part_upd, part_apply,
-                                                                           (int)nblk_stencil);
+                                                                           (int)nblk_apply);
                     else
                         k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr,
                                                                       nullptr, st, part_upd, part_apply,
-                                                                      (int)nblk_stencil);
+                                                                      (int)nblk_apply);
                     HIPTRY(hipGetLastError());
                     return TH_OK;
                 }
