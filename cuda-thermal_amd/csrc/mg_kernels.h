@@ -24,8 +24,9 @@ enum { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 constexpr int kTailThreads = 512;
 constexpr int kMaxTail = 8;
 constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
-constexpr size_t kTailLdsBytes = 150 * 1024;  // LDS budget of the LDS-resident tail (+ 5 or 9 KB static)
+constexpr size_t kTailLdsBytes = 148 * 1024;  // LDS budget of the LDS-resident tail (+ 5 or 10 KB static)
 constexpr int kTailLdsThreads = 1024;           // plane-parallel sweeps: 64 / LP columns per wave
+constexpr int kTailScratch = kTailLdsThreads + 64;  // >= LP * (kTailLdsThreads / LP + 1) for LP in {4, 8, 16}
 
 // Coarse parents of the 4 lateral neighbours of column (i, j) (SW_PROLONG).
 struct Parents {
@@ -148,11 +149,13 @@ __device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const 
                                               const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
                                               const T* __restrict__ xc, State* st,
                                               const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
-    __shared__ __align__(16) T sh_s[kPPThreads], sh_i[kPPThreads], sh_u[kPPThreads];
+    // Scratch rows by plane, CPB + 1 apart: the Thomas lanes (one per
+    // column) read consecutive words instead of words LP apart in one bank.
+    constexpr int CPB = kPPThreads / LP, RS = CPB + 1;
+    __shared__ __align__(16) T sh_s[LP * RS], sh_i[LP * RS], sh_u[LP * RS];
     __shared__ int sh_ok[kPPThreads / 4];
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
-    constexpr int CPB = kPPThreads / LP;
     const int tid = threadIdx.x, l = tid % LP, cb = tid / LP;
     const long q = (long)B.x * CPB + cb;  // column of this colour: t + hx * j
     const int j = (int)(q / g.hx), t = (int)(q - (long)j * g.hx);
@@ -164,27 +167,26 @@ __device__ __forceinline__ void sweep_pp_body(const Blk& B, const Geo& g, const 
     if (ok) {
         const Parents pa = MODE == SW_PROLONG ? parents(gc, c, i, j) : Parents{0, 0, 0, 0};
         const long id = (long)l * g.plane + c.c0;
-        sh_s[tid] = column_rhs<T, MODE>(g, o.gE, o.gN, b, x, c, l, gc, xc, pa);
-        sh_i[tid] = finv[id];
-        sh_u[tid] = o.gU[id];
+        sh_s[l * RS + cb] = column_rhs<T, MODE>(g, o.gE, o.gN, b, x, c, l, gc, xc, pa);
+        sh_i[l * RS + cb] = finv[id];
+        sh_u[l * RS + cb] = o.gU[id];
     }
     if (l == 0) sh_ok[cb] = colok;
     __syncthreads();
     if (tid < CPB && sh_ok[tid]) {
-        const int o0 = tid * LP;
         T s[LP], inv[LP], gu[LP];
 #pragma unroll
         for (int k = 0; k < LP; ++k) {
-            s[k] = sh_s[o0 + k];
-            inv[k] = sh_i[o0 + k];
-            gu[k] = sh_u[o0 + k];
+            s[k] = sh_s[k * RS + tid];
+            inv[k] = sh_i[k * RS + tid];
+            gu[k] = sh_u[k * RS + tid];
         }
         column_thomas<T, LP>(g.nz, s, inv, gu);
 #pragma unroll
-        for (int k = 0; k < LP; ++k) sh_s[o0 + k] = s[k];
+        for (int k = 0; k < LP; ++k) sh_s[k * RS + tid] = s[k];
     }
     __syncthreads();
-    if (ok) x[(long)l * g.plane + c.c0] = sh_s[tid];
+    if (ok) x[(long)l * g.plane + c.c0] = sh_s[l * RS + cb];
 }
 template <class T, int LP, int MODE>
 __global__ __launch_bounds__(kPPThreads) void k_sweep_pp(Geo g, Ops<T> o, const T* __restrict__ finv,
@@ -574,12 +576,25 @@ __global__ __launch_bounds__((tail_threads<T, NZ>())) void k_tail(Tail<T> t, Sta
     tail_vcycle<T, NZ, TT>(t.L, t.n, (T)st->inv_dt, fast);
 }
 
-// LDS bytes of an LDS-resident tail over levels g[0..n-1]: per cell gE, gN,
-// gU, 1/pivot, x, b (the lateral restriction reads no gB and no capacity).
+// A level copied into the LDS tail keeps its layout except for the plane
+// stride: the smallest stride >= the used slots that is 64 / lp modulo 32
+// words. The lp lanes of a column read lp planes of it at once; with a
+// stride of 2 hx ny (256 words at 16 x 16) they would all fall in one bank,
+// with this one the 64 lanes of a wave (64 / lp columns) fall in distinct
+// banks pairwise at most.
+__host__ __device__ inline Geo tail_lds_geo(Geo g, int lp) {
+    const long slots = 2L * g.hx * g.ny, r = (64 / lp) % 32;
+    g.plane = slots + (((r - slots) % 32) + 32) % 32;
+    return g;
+}
+
+// LDS bytes of an LDS-resident tail over levels g[0..n-1] (lp lanes per
+// column): per cell gE, gN, gU, 1/pivot, x, b (the lateral restriction reads
+// no gB and no capacity).
 template <class T>
-inline size_t tail_lds_bytes(const Geo* g, int n) {
+inline size_t tail_lds_bytes(const Geo* g, int n, int lp) {
     size_t b = 0;
-    for (int k = 0; k < n; ++k) b += 6 * sizeof(T) * (size_t)g[k].cells();
+    for (int k = 0; k < n; ++k) b += 6 * sizeof(T) * (size_t)tail_lds_geo(g[k], lp).cells();
     return b;
 }
 
@@ -596,10 +611,10 @@ struct LdsLevel {
     T *gE, *gN, *gU, *fi, *x, *b;
 };
 template <class T>
-__device__ __forceinline__ LdsLevel<T> lds_level(const Tail<T>& t, T* sm, int k) {
+__device__ __forceinline__ LdsLevel<T> lds_level(const Tail<T>& t, T* sm, int k, int lp) {
     size_t off = 0;
-    for (int m = 0; m < k; ++m) off += 6 * (size_t)t.L[m].g.cells();
-    const Geo g = t.L[k].g;
+    for (int m = 0; m < k; ++m) off += 6 * (size_t)tail_lds_geo(t.L[m].g, lp).cells();
+    const Geo g = tail_lds_geo(t.L[k].g, lp);
     const long cells = g.cells();
     T* p = sm + off;
     return LdsLevel<T>{g, p, p + cells, p + 2 * cells, p + 3 * cells, p + 4 * cells, p + 5 * cells};
@@ -609,7 +624,8 @@ template <class T, int LP, int MODE>
 __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s,
                                               int* scr_c) {
     constexpr int CPP = kTailLdsThreads / LP;  // columns per pass
-    const int tid = threadIdx.x, l = tid % LP;
+    constexpr int RS = CPP + 1;                 // scratch row stride (rows by plane, see sweep_pp_body)
+    const int tid = threadIdx.x, l = tid % LP, cp = tid / LP;
     const Geo& g = F.g;
     const int ncols = g.hx * g.ny;
     for (int q0 = 0; q0 < ncols; q0 += CPP) {
@@ -622,19 +638,19 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
         if (colok) c = make_col(g, i, j);
         if (ok) {
             const Parents pa = MODE == SW_PROLONG ? parents(C.g, c, i, j) : Parents{0, 0, 0, 0};
-            scr_s[tid] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
+            scr_s[l * RS + cp] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
         }
-        if (l == 0) scr_c[tid / LP] = colok ? c.c0 : -1;
+        if (l == 0) scr_c[cp] = colok ? c.c0 : -1;
         __syncthreads();
         // Thread k < CPP solves column k of the pass (one wave at LP = 16).
         const int c0 = tid < CPP ? scr_c[tid] : -1;
         if (c0 >= 0) {
-            T* sc = scr_s + tid * LP;
+            T* sc = scr_s + tid;  // plane k at sc[k * RS]
             if constexpr (sizeof(T) == 4) {
                 T sv[LP], iv[LP], uv[LP];
 #pragma unroll
                 for (int k = 0; k < LP; ++k) {
-                    sv[k] = sc[k];
+                    sv[k] = sc[k * RS];
                     if (k < g.nz) {
                         const long id = (long)k * g.plane + c0;
                         iv[k] = F.fi[id];
@@ -643,7 +659,7 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
                 }
                 column_thomas<T, LP>(g.nz, sv, iv, uv);
 #pragma unroll
-                for (int k = 0; k < LP; ++k) sc[k] = sv[k];
+                for (int k = 0; k < LP; ++k) sc[k * RS] = sv[k];
             } else {
                 // Three FP64 columns of 16 planes exceed a 1024-thread
                 // block's 128 registers: the recurrence (column_thomas's
@@ -653,17 +669,17 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
                 for (int k = 0; k < nz; ++k) {
                     const long id = (long)k * g.plane + c0;
                     const T lower = (k > 0) ? -F.gU[id - g.plane] : T(0);
-                    prev = (sc[k] - lower * prev) * F.fi[id];
-                    sc[k] = prev;
+                    prev = (sc[k * RS] - lower * prev) * F.fi[id];
+                    sc[k * RS] = prev;
                 }
                 for (int k = nz - 2; k >= 0; --k) {
                     const long id = (long)k * g.plane + c0;
-                    sc[k] = sc[k] - ((-F.gU[id]) * F.fi[id]) * sc[k + 1];
+                    sc[k * RS] = sc[k * RS] - ((-F.gU[id]) * F.fi[id]) * sc[(k + 1) * RS];
                 }
             }
         }
         __syncthreads();
-        if (ok) F.x[(long)l * g.plane + c.c0] = scr_s[tid];
+        if (ok) F.x[(long)l * g.plane + c.c0] = scr_s[l * RS + cp];
     }
     __syncthreads();
 }
@@ -672,26 +688,28 @@ template <class T, int NZ>
 __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const double* __restrict__ cons_part,
                                               int cons_nblk, int cons_mode, T* sm) {
     constexpr int LP = pp_lanes(NZ);
-    __shared__ __align__(16) T scr_s[kTailLdsThreads];
+    __shared__ __align__(16) T scr_s[kTailScratch];
     __shared__ int scr_c[kTailLdsThreads / 4];  // pass columns' c0 (-1: none)
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
     for (int k = 0; k < t.n; ++k) {
         const TailLevel<T>& G = t.L[k];
-        const LdsLevel<T> D = lds_level<T>(t, sm, k);
+        const LdsLevel<T> D = lds_level<T>(t, sm, k, LP);
         const long cells = G.g.cells();
         for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
-            D.gE[q] = G.o.gE[q];
-            D.gN[q] = G.o.gN[q];
-            D.gU[q] = G.o.gU[q];
-            D.fi[q] = G.finv[q];
-            if (k == 0) D.b[q] = G.b[q];
+            const long l = q / G.g.plane, sl = q - l * G.g.plane, d = l * D.g.plane + sl;
+            if (sl >= D.g.slots()) continue;  // pad slots are never read
+            D.gE[d] = G.o.gE[q];
+            D.gN[d] = G.o.gN[q];
+            D.gU[d] = G.o.gU[q];
+            D.fi[d] = G.finv[q];
+            if (k == 0) D.b[d] = G.b[q];
         }
     }
     __syncthreads();
     const int n = t.n;
     for (int k = 0; k + 1 < n; ++k) {
-        const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
+        const LdsLevel<T> F = lds_level<T>(t, sm, k, LP), C = lds_level<T>(t, sm, k + 1, LP);
         tail_sweep_pp<T, LP, SW_ZERO>(F, 0, C, scr_s, scr_c);
         tail_sweep_pp<T, LP, SW_PLAIN>(F, 1, C, scr_s, scr_c);
         const int cells = C.g.nx * C.g.ny * C.g.nz;
@@ -702,17 +720,21 @@ __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const
         __syncthreads();
     }
     {
-        const LdsLevel<T> N = lds_level<T>(t, sm, n - 1);
+        const LdsLevel<T> N = lds_level<T>(t, sm, n - 1, LP);
         tail_sweep_pp<T, LP, SW_ZERO>(N, 0, N, scr_s, scr_c);
     }
     for (int k = n - 2; k >= 0; --k) {
-        const LdsLevel<T> F = lds_level<T>(t, sm, k), C = lds_level<T>(t, sm, k + 1);
+        const LdsLevel<T> F = lds_level<T>(t, sm, k, LP), C = lds_level<T>(t, sm, k + 1, LP);
         tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s, scr_c);
         tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s, scr_c);
     }
-    const LdsLevel<T> T0 = lds_level<T>(t, sm, 0);
-    const long cells0 = t.L[0].g.cells();
-    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) t.L[0].x[q] = T0.x[q];
+    const LdsLevel<T> T0 = lds_level<T>(t, sm, 0, LP);
+    const Geo G0 = t.L[0].g;
+    const long cells0 = G0.cells();
+    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) {
+        const long l = q / G0.plane, sl = q - l * G0.plane;
+        t.L[0].x[q] = sl < G0.slots() ? T0.x[l * T0.g.plane + sl] : T(0);
+    }
 }
 template <class T, int NZ>
 __global__ __launch_bounds__(kTailLdsThreads) void k_tail_lds(Tail<T> t, State* st, const double* __restrict__ cons_part,

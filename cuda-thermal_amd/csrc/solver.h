@@ -260,7 +260,7 @@ struct Solver final : SolverBase {
             auto bytes_from = [&](size_t k0) {
                 std::vector<Geo> gs;
                 for (size_t k = k0; k < L.size(); ++k) gs.push_back(L[k].g);
-                return tail_lds_bytes<T>(gs.data(), (int)gs.size());
+                return tail_lds_bytes<T>(gs.data(), (int)gs.size(), pp_lanes(g0.nz));
             };
             while (tail_k0 > 0 && bytes_from(tail_k0 - 1) <= kTailLdsBytes && L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
                 --tail_k0;
