@@ -81,6 +81,7 @@ struct State {
     double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
     int maxit, iters, done, converged, zero_x, flexible;
     int xlag;  // multigrid path: x += alpha p of the last update is still to be applied
+    int xz;    // x is known to be 0 at the start of this solve: its residual is b
     // Transient stepping on the device (transient3d.h:57-84): the time, the
     // step size of the current operator and of the next step, the stepping
     // rule, the heat content after the last step and that step's record.

@@ -42,13 +42,17 @@ __device__ __forceinline__ void residual_body(const Blk& B, const Geo& g, const 
     const int j = B.y * blockDim.y + threadIdx.y;
     double v[3] = {0.0, 0.0, 0.0};  // rr, bb, rz
     const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
+    // x = 0 (multigrid start after set_x_zero): every flux term is +-0, so
+    // r = b exactly and x is not read.
+    const bool xz = MODE == RES_MG && st->xz;
     int i;
     if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
         const Col c = make_col(g, i, j);
         for (int l = 0; l < g.nz; ++l) {
             const long id = (long)l * g.plane + c.c0;
             const double bi = b[id];
-            const double ri = bi - flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, x, c, l, inv_dt);
+            double ri = bi;
+            if (!xz) ri = bi - flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, x, c, l, inv_dt);
             v[0] += ri * ri;
             v[1] += bi * bi;
             if (MODE == RES_MG) {
@@ -469,8 +473,9 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
 }
 
 // Start of a solve: tolerance, iteration cap, cleared flags.
-static __global__ void k_begin(State* st, double tol, int maxit) {
+static __global__ void k_begin(State* st, double tol, int maxit, int xz) {
     st->xlag = 0;
+    st->xz = xz;
     st->tol = tol;
     st->maxit = maxit;
     st->iters = 0;
@@ -482,9 +487,11 @@ static __global__ void k_begin(State* st, double tol, int maxit) {
 // Reopens a finished solve for a restart from the current x (iterations kept).
 static __global__ void k_reopen(State* st) {
     st->xlag = 0;
+    st->xz = 0;
     st->done = 0;
     st->converged = 0;
 }
+static __global__ void k_clear_xz(State* st) { st->xz = 0; }
 static __global__ void k_set_inv_dt(State* st, double inv_dt) {
     st->inv_dt = inv_dt;
     st->refactor = 1;
@@ -510,6 +517,7 @@ static __global__ void k_step_init(State* st, double dt0, double growth, double 
 // Then tolerance and flags of the step's solve, as k_begin.
 static __global__ void k_step_begin(State* st, double tol, int maxit) {
     st->xlag = 0;
+    st->xz = 0;
     const double dt = fmin(st->dt_next, st->t_end - st->t);
     st->dt_prev = st->dt;
     st->refactor = dt != st->dt;

@@ -643,19 +643,47 @@ struct Solver final : SolverBase {
         return enqueue_iter();
     }
 
+    // Residual replacement: r := b - A x, the Krylov state (p, rz, beta)
+    // kept. The recursive residual drifts from the true one by rounding in
+    // x += alpha p and r -= alpha Ap, most of it while x still moves a lot;
+    // at 1024^2 x 16 the true relative residual of a solve stalls near 1e-10
+    // however far the recursive one goes, and the true-residual check then
+    // restarts the solve (a start V-cycle and the Krylov history lost).
+    // Replacing r once, when the recursive relative residual first falls
+    // below kReplaceAt * tol, leaves a gap far below tol.
+    static constexpr double kReplaceAt = 1e4;
+    int replace_residual() {
+        k_clear_xz<<<1, 1, 0, s>>>(st);  // x is no longer 0
+        HIPTRY(hipGetLastError());
+        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
+        k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
+                                                               nullptr, st, part_res);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
     // PCG iterations until the device reports done. Iterations are queued in
     // batches sized from the observed convergence rate, so the host waits
     // once per batch rather than once per iteration; iterations queued past
     // convergence return at once. `first`: size of the first batch.
-    int iterate(int maxit, int first = 1) {
+    // `replace`: one residual replacement on the way (multigrid solves with
+    // the true-residual check, tol < 1e-8); the batch before it stops where
+    // the replacement is due. After an iteration x is complete (the p update
+    // applies the lagged x += alpha p).
+    int iterate(int maxit, int first = 1, bool replace = false) {
         int rc;
         double rate = 0.0;  // relres reduction per iteration, once known
+        replace = replace && opt.true_residual && mg() && h_st->tol < 1e-8;
         while (!h_st->done && h_st->iters < maxit) {
             int k = first;
             first = 1;
             if (rate > 0.0 && rate < 1.0 && h_st->relres > 0.0) {
                 const double need = std::log(h_st->tol / h_st->relres) / std::log(rate);
                 k = (int)std::max(1.0, std::min(16.0, std::ceil(need) - 1.0));
+                if (replace && h_st->relres >= kReplaceAt * h_st->tol) {
+                    const double nr = std::log(kReplaceAt * h_st->tol / h_st->relres) / std::log(rate);
+                    k = std::min(k, (int)std::max(1.0, std::ceil(nr)));
+                }
             }
             k = std::min(k, maxit - h_st->iters);
             const int it0 = h_st->iters;
@@ -666,14 +694,19 @@ struct Solver final : SolverBase {
             const int did = h_st->iters - it0;
             if (did > 0 && rel0 > 0.0 && h_st->relres > 0.0) rate = std::pow(h_st->relres / rel0, 1.0 / did);
             if (did == 0) break;  // nothing ran: the device is done
+            if (replace && !h_st->done && h_st->relres < kReplaceAt * h_st->tol) {
+                if ((rc = replace_residual())) return rc;
+                replace = false;
+            }
         }
         return TH_OK;
     }
 
     int solve(double tol, int maxit, th_stats* out) override {
         int rc;
+        const int xz = x_zero ? 1 : 0;  // the start residual is b
         x_zero = false;
-        k_begin<<<1, 1, 0, s>>>(st, tol, maxit);
+        k_begin<<<1, 1, 0, s>>>(st, tol, maxit, xz);
         HIPTRY(hipGetLastError());
         if ((rc = run_start()) || (rc = fetch_state())) return rc;
         if ((rc = converge(maxit, 1))) return rc;
@@ -693,12 +726,21 @@ struct Solver final : SolverBase {
     // below tol too.
     int converge(int maxit, int first) {
         int rc;
-        if (maxit > 0 && (rc = iterate(maxit, first))) return rc;
+        if (maxit > 0 && (rc = iterate(maxit, first, true))) return rc;
         for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
             k_finish<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, pm, st);  // x complete
             HIPTRY(hipGetLastError());
             k_reopen<<<1, 1, 0, s>>>(st);
             HIPTRY(hipGetLastError());
+            // The true residual of x and the convergence test on it, in one
+            // kernel and a one-block finalize: when it is below tol the start
+            // graph (a V-cycle's worth of launches) is not replayed.
+            const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
+            k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
+                                                                   nullptr, st, part_res);
+            HIPTRY(hipGetLastError());
+            if ((rc = finalize<3, FIN_RES>(nblk_stencil, part_res)) || (rc = fetch_state())) return rc;
+            if (h_st->done) break;
             if ((rc = run_start()) || (rc = fetch_state())) return rc;
             if (h_st->done) break;  // true residual below tol
             if (h_st->iters >= maxit) break;
@@ -910,7 +952,7 @@ struct Solver final : SolverBase {
     int precondition(const double* host_r, double* host_z) override {
         int rc;
         if ((rc = upload_split<double, double>(g0, g0.nz, host_r, r))) return rc;
-        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1);
+        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1, 0);
         HIPTRY(hipGetLastError());
         const unsigned vb = vec_blocks(n0);
         double* zd = Ap;  // z as FP64 in the split layout
@@ -928,7 +970,7 @@ struct Solver final : SolverBase {
     int apply_operator(const double* host_x, double* host_y) override {
         int rc;
         if ((rc = upload_split<double, double>(g0, g0.nz, host_x, p))) return rc;
-        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1);
+        k_begin<<<1, 1, 0, s>>>(st, 0.0, 1, 0);
         HIPTRY(hipGetLastError());
         if ((rc = launch_apply())) return rc;
         return download_natural(Ap, host_y);
