@@ -78,6 +78,100 @@ __global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double
     residual_body<T, MODE>(hw_blk(), g, o, b, x, r, b0, invd, z, p, st, part);
 }
 
+// residual_body for a level with column types (k_residual_typed): the same
+// flux-form arithmetic per cell, walking the column with the planes below and
+// above in registers and the faces from the column's type (UNI: through the
+// scalar cache). Absent lateral faces add (+0) * dx: f starts at +0 and never
+// becomes -0, so the sums are flux_apply_cell's. XZ: x = 0, r = b (x is not
+// read). The per-column sums are k_residual's; the blocks are 64 x 4.
+template <class T, int NZ, int MODE, bool TR, bool UNI, bool XZ>
+__device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<T>& y, const double* __restrict__ b,
+                                                      const double* __restrict__ x, double* __restrict__ r,
+                                                      T* __restrict__ b0, const Col& c, int t, double inv_dt,
+                                                      double (&v)[3]) {
+    const int nz = g.nz;
+    if (XZ) {
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) {
+            const int lc = l < nz ? l : nz - 1;
+            const long id = (long)lc * g.plane + c.c0;
+            const double bi = b[id];
+            if (plane_live<NZ>(l, nz)) {
+                v[0] += bi * bi;
+                v[1] += bi * bi;
+                if (MODE == RES_MG) {
+                    r[id] = bi;
+                    b0[id] = (T)bi;
+                }
+            }
+        }
+        return;
+    }
+    double xd = 0.0, xc = x[c.c0];
+    double gdn = (double)(UNI ? y.urow(t, 0) : y.row(t, 0)).v[TF_B];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        const int lc = l < nz ? l : nz - 1;
+        const bool up = l < nz - 1;
+        const long base = (long)lc * g.plane, id = base + c.c0;
+        const TypeRow<T> rw = UNI ? y.urow(t, lc) : y.row(t, lc);
+        const double bi = b[id];
+        const double xu = x[id + (up ? g.plane : 0)];
+        const double xw = x[base + c.pw], xe = x[base + c.pe], xs = x[base + c.ps], xn = x[base + c.pn];
+        const double Gw = c.hw ? (double)rw.v[TF_W] : 0.0, Ge = c.he ? (double)rw.v[TF_E] : 0.0;
+        const double Gs = c.hs ? (double)rw.v[TF_S] : 0.0, Gn = c.hn ? (double)rw.v[TF_N] : 0.0;
+        const double Gu = (double)rw.v[TF_U];
+        double f = 0.0;
+        f += Gw * (xc - xw);
+        f += Ge * (xc - xe);
+        f += Gs * (xc - xs);
+        f += Gn * (xc - xn);
+        f += (l > 0) ? gdn * (xc - xd) : gdn * xc;  // below, or bottom ambient
+        f += up ? Gu * (xc - xu) : Gu * xc;         // above, or the heat sink
+        if (TR) f += (double)rw.v[TF_C] * inv_dt * xc;
+        const double ri = bi - f;
+        if (plane_live<NZ>(l, nz)) {
+            v[0] += ri * ri;
+            v[1] += bi * bi;
+            if (MODE == RES_MG) {
+                r[id] = ri;
+                b0[id] = (T)ri;
+            }
+        }
+        xd = xc;
+        xc = xu;
+        gdn = Gu;
+    }
+}
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
+                                                         const double* __restrict__ x, double* __restrict__ r,
+                                                         T* __restrict__ b0, const State* __restrict__ st,
+                                                         double* __restrict__ part) {
+    if (MODE != RES_CHECK && st->done) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double v[3] = {0.0, 0.0, 0.0};  // rr, bb, (rz)
+    const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
+    const bool xz = MODE == RES_MG && st->xz;
+    int i;
+    if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
+        const Col c = make_col(g, i, j);
+        const int ty = y.ty[c.c0];
+        const int rep = wave_rep_type(ty);
+        if (xz)
+            residual_column_typed<T, NZ, MODE, false, true, true>(g, y, b, x, r, b0, c, 0, inv_dt, v);
+        else if (__all(ty == rep)) {
+            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v);
+            else residual_column_typed<T, NZ, MODE, false, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v);
+        } else {
+            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v);
+            else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v);
+        }
+    }
+    block_partials<3>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+}
+
 // Ap = (A + C/dt) p; partial p.Ap.
 // One column of Ap and its p.Ap terms. TR: the C/dt term is in the diagonal
 // (steady solves never load the capacities).

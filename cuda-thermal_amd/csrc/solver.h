@@ -242,11 +242,14 @@ struct Solver final : SolverBase {
         if ((rc = dalloc(&part, maxblk * 4))) return rc;
         {
             const dim3 sg = stencil_grid(2 * g0.hx, g0.ny);
-            const dim3 ag = L[0].ntypes ? col_grid(2 * g0.hx, g0.ny) : sg;  // launch_apply_kernel's grid
+            // launch_apply_kernel's and launch_residual's grid: 64 x 4 blocks with column types
+            const dim3 ag = L[0].ntypes ? col_grid(2 * g0.hx, g0.ny) : sg;
             nblk_stencil = sg.x * sg.y;
             nblk_apply = ag.x * ag.y;
+            nblk_res = ag.x * ag.y;
             nblk_vec = vec_blocks(n0);
-            if ((rc = dalloc(&part_res, (size_t)nblk_stencil * 3)) || (rc = dalloc(&part_apply, nblk_apply)) ||
+            if ((rc = dalloc(&part_res, (size_t)std::max(nblk_stencil, nblk_res) * 3)) ||
+                (rc = dalloc(&part_apply, nblk_apply)) ||
                 (rc = dalloc(&part_upd, (size_t)nblk_vec * 2)) || (rc = dalloc(&part_dot, nblk_vec)))
                 return rc;
         }
@@ -381,7 +384,7 @@ struct Solver final : SolverBase {
     // Partial buffers, one per producing kernel, so a consumer can still read
     // its producer's partials while the next reduction runs.
     double *part_res = nullptr, *part_apply = nullptr, *part_upd = nullptr, *part_dot = nullptr;
-    unsigned nblk_stencil = 0, nblk_apply = 0, nblk_vec = 0;
+    unsigned nblk_stencil = 0, nblk_apply = 0, nblk_res = 0, nblk_vec = 0;
 
     template <int NV, int EPI, unsigned KMAX = 0>
     int finalize(unsigned nblk, const double* pp = nullptr) {
@@ -567,16 +570,42 @@ struct Solver final : SolverBase {
         return finalize<1, FIN_ALPHA>(nblk_apply, part_apply);
     }
 
+    // r = b - (A + C/dt) x with b0 = (T) r (RES_MG), or the steady ||P - A v||
+    // (RES_CHECK, no writes), nblk_res partials into pp: the typed column
+    // walk (64 x 4 blocks) when the fine level has column types.
+    template <int MODE>
+    int launch_residual(const double* bb, const double* xx, double* pp) {
+        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
+        double* rr = MODE == RES_MG ? r : nullptr;
+        T* b0 = MODE == RES_MG ? L[0].b : nullptr;
+        if (L[0].ntypes) {
+            const dim3 tg = col_grid(2 * g0.hx, g0.ny);
+#define TH_RES(NZ) k_residual_typed<T, NZ, MODE><<<tg, col_block(), 0, s>>>(g0, L[0].typed(), bb, xx, rr, b0, st, pp)
+            if (g0.nz <= 4) TH_RES(4);
+            else if (g0.nz <= 8) TH_RES(8);
+            else if (g0.nz <= 12) TH_RES(12);
+            else if (g0.nz <= 16) TH_RES(16);
+            else if (g0.nz <= 24) TH_RES(24);
+            else if (g0.nz <= 32) TH_RES(32);
+            else if (g0.nz <= 48) TH_RES(48);
+            else TH_RES(64);
+#undef TH_RES
+        } else {
+            k_residual<T, MODE><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), bb, xx, rr, b0, nullptr, nullptr,
+                                                                 nullptr, st, pp);
+        }
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
     // Start: residual, first preconditioner application, rz, p = z.
     int enqueue_start() {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const unsigned vb = vec_blocks(n0);
         int rc;
         if (mg()) {
-            k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
-                                                                   nullptr, st, part_res);
-            HIPTRY(hipGetLastError());
-            if ((rc = vcycle(0, Cons{part_res, (int)nblk_stencil, CONS_START}))) return rc;
+            if ((rc = launch_residual<RES_MG>(b, x, part_res))) return rc;
+            if ((rc = vcycle(0, Cons{part_res, (int)nblk_res, CONS_START}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
             k_pupdate<T, 1, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, pm, nullptr, st, part_dot, (int)vb);
@@ -655,11 +684,7 @@ struct Solver final : SolverBase {
     int replace_residual() {
         k_clear_xz<<<1, 1, 0, s>>>(st);  // x is no longer 0
         HIPTRY(hipGetLastError());
-        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
-                                                               nullptr, st, part_res);
-        HIPTRY(hipGetLastError());
-        return TH_OK;
+        return launch_residual<RES_MG>(b, x, part_res);
     }
 
     // PCG iterations until the device reports done. Iterations are queued in
@@ -735,11 +760,9 @@ struct Solver final : SolverBase {
             // The true residual of x and the convergence test on it, in one
             // kernel and a one-block finalize: when it is below tol the start
             // graph (a V-cycle's worth of launches) is not replayed.
-            const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-            k_residual<T, RES_MG><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, L[0].b, nullptr, nullptr,
-                                                                   nullptr, st, part_res);
-            HIPTRY(hipGetLastError());
-            if ((rc = finalize<3, FIN_RES>(nblk_stencil, part_res)) || (rc = fetch_state())) return rc;
+            if ((rc = launch_residual<RES_MG>(b, x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
+                (rc = fetch_state()))
+                return rc;
             if (h_st->done) break;
             if ((rc = run_start()) || (rc = fetch_state())) return rc;
             if (h_st->done) break;  // true residual below tol
@@ -925,12 +948,10 @@ struct Solver final : SolverBase {
     }
     // ||P - A v|| / ||P|| of the steady operator (flux form)
     int relres_of(const double* v, double* out) {
-        const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        k_residual<T, RES_CHECK><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), P, v, nullptr, nullptr, nullptr,
-                                                                  nullptr, nullptr, st, part);
-        HIPTRY(hipGetLastError());
         int rc;
-        if ((rc = finalize<3, FIN_CHECK>(grid.x * grid.y)) || (rc = fetch_state())) return rc;
+        if ((rc = launch_residual<RES_CHECK>(P, v, part)) || (rc = finalize<3, FIN_CHECK>(nblk_res)) ||
+            (rc = fetch_state()))
+            return rc;
         *out = std::sqrt(h_st->chk_rr / h_st->chk_bb);
         return TH_OK;
     }

@@ -147,3 +147,35 @@ def test_operator_bitwise_large_levels(nz, precision, dt):
     o.set_dt(dt)
     x = np.random.default_rng(nz).standard_normal(m.n)
     assert np.array_equal(s.apply(x), o.apply(x))
+
+
+@pytest.mark.parametrize("nz", [1, 6, 16])
+@pytest.mark.parametrize("x0", ["zero", "random"])
+def test_true_residual_large_levels(nz, x0):
+    """||P - A x|| / ||P|| on a grid with column types (the typed column-walk
+    residual, flux form) against the oracle's flux-form residual, for x = 0
+    and for a random x; then a steady solve on the same grid reaches a true
+    relative residual <= 1e-10 through the typed start, replacement and
+    check residuals."""
+    layers = STACKS[nz] if nz != 16 else [SI + (4, 1), BOND + (2, 0), SI + (4, 1), TIM + (2, 0), CU + (4, 0)]
+    m = T.Model.from_layers(layers, 301, 171)
+    m.add_uniform_power(0, 20.0)
+    m.add_hotspot(0, 3e-3, 4e-3, 1e-3, 3.0)
+    m.add_tsv_region(0, 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
+    m.finalize()
+    a = m.arrays()
+    s = T.Solver(m, precision="mixed", true_residual=True)
+    assert s.column_types() > 0
+    x = np.zeros(m.n) if x0 == "zero" else 40.0 + np.random.default_rng(nz).standard_normal(m.n)
+    s.set_x(x)
+    tr, ref = s.true_relres(), O.true_relres_flux(a, x)
+    # the norms are summed in another order than the oracle's (1e-12 of a
+    # residual norm of ~3e3 for the random x); a wrong face would be O(1)
+    assert abs(tr - ref) <= 1e-10 * ref, (tr, ref)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st = s.solve(1e-10, 200)
+    assert st["converged"]
+    xs = s.get_x()
+    assert O.true_relres_flux(a, xs) <= 1.5e-10
+    assert abs(s.true_relres() - O.true_relres_flux(a, xs)) <= 1e-6 * s.true_relres()
