@@ -883,6 +883,68 @@ __global__ __launch_bounds__(256) void k_line_relax(Geo g, Ops<T> o, const doubl
     }
 }
 
+// k_line_relax on a level with column types: the faces from the column's
+// type (UNI: wave-uniform, through the scalar cache), absent lateral faces
+// masked to +0 (d and rhs start at +0 or b and never become -0, so the sums
+// are k_line_relax's), every plane's loads unconditional; the same Thomas
+// arithmetic.
+template <class T, int NZ, bool UNI>
+__device__ __forceinline__ void line_relax_typed_column(const Geo& g, const Typed<T>& y, const double* __restrict__ b,
+                                                        double* __restrict__ x, const Col& c, int t) {
+    const int nz = g.nz;
+    double cp[NZ], dp[NZ];
+    double pcp = 0.0, pdp = 0.0;
+    double gdn = (double)(UNI ? y.urow(t, 0) : y.row(t, 0)).v[TF_B];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        const int lc = l < nz ? l : nz - 1;
+        const long base = (long)lc * g.plane, id = base + c.c0;
+        const TypeRow<T> rw = UNI ? y.urow(t, lc) : y.row(t, lc);
+        const double Gw = c.hw ? (double)rw.v[TF_W] : 0.0, Ge = c.he ? (double)rw.v[TF_E] : 0.0;
+        const double Gs = c.hs ? (double)rw.v[TF_S] : 0.0, Gn = c.hn ? (double)rw.v[TF_N] : 0.0;
+        const double Gu = (double)rw.v[TF_U], Gd = gdn;
+        const double xw = x[base + c.pw], xe = x[base + c.pe], xs = x[base + c.ps], xn = x[base + c.pn];
+        double d = 0.0, rhs = b[id];
+        d += Gw, rhs += Gw * xw;
+        d += Ge, rhs += Ge * xe;
+        d += Gs, rhs += Gs * xs;
+        d += Gn, rhs += Gn * xn;
+        d += Gu + Gd;
+        const double lower = (l > 0) ? -Gd : 0.0, upper = (l < nz - 1) ? -Gu : 0.0;
+        const double den = d - lower * pcp;
+        cp[l] = upper / den;
+        dp[l] = (rhs - lower * pdp) / den;
+        pcp = cp[l];
+        pdp = dp[l];
+        gdn = Gu;
+    }
+    double xu = 0.0;
+#pragma unroll
+    for (int l = NZ - 1; l >= 0; --l) {
+        if (l < nz) {
+            const double v = (l == nz - 1) ? dp[l] : dp[l] - cp[l] * xu;
+            x[(long)l * g.plane + c.c0] = v;
+            xu = v;
+        }
+    }
+}
+template <class T, int NZ>
+__global__ __launch_bounds__(256) void k_line_relax_typed(Geo g, Typed<T> y, const double* __restrict__ b,
+                                                          double* __restrict__ x, int colour) {
+    const int t = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    if (j >= g.ny || t >= g.hx) return;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= g.nx) return;
+    const Col c = make_col(g, i, j);
+    const int ty = y.ty[c.c0];
+    const int rep = wave_rep_type(ty);
+    if (__all(ty == rep))
+        line_relax_typed_column<T, NZ, true>(g, y, b, x, c, rep);
+    else
+        line_relax_typed_column<T, NZ, false>(g, y, b, x, c, ty);
+}
+
 // Natural <-> split layout. Padding slots of `out` are left untouched
 // (they were cleared at allocation).
 template <class S, class D>

@@ -883,7 +883,13 @@ struct Solver final : SolverBase {
         for (int k = 0; k < passes; ++k)
             for (int colour = 0; colour < 2; ++colour) {
                 const dim3 grid = col_grid(g0.hx, g0.ny);
-#define TH_RELAX(NZ) k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, v, colour)
+#define TH_RELAX(NZ)                                                                                       \
+    do {                                                                                                   \
+        if (L[0].ntypes)                                                                                   \
+            k_line_relax_typed<T, NZ><<<grid, col_block(), 0, s>>>(g0, L[0].typed(), P, v, colour);       \
+        else                                                                                               \
+            k_line_relax<T, NZ><<<grid, col_block(), 0, s>>>(g0, o, P, v, colour);                        \
+    } while (0)
                 if (g0.nz <= 4) TH_RELAX(4);
                 else if (g0.nz <= 8) TH_RELAX(8);
                 else if (g0.nz <= 12) TH_RELAX(12);
