@@ -123,8 +123,12 @@ typedef struct {
     int use_graphs; /* replay iterations from hipGraphs (gpu_use_graphs, pcg3d.h:167) */
     int device;     /* HIP device ordinal */
     int true_residual; /* 1: on convergence recompute b - A x in FP64 and restart
-                          from x while it is above tol (up to 4 restarts); 0: stop
-                          on the recurrence residual, as the reference does */
+                          from x while it is above tol (up to 4 restarts); with the
+                          multigrid preconditioner and tol < 1e-8 the recurrence
+                          residual is also replaced once by b - A x on the way (when
+                          it falls below 1e4 tol), which keeps the two close enough
+                          that the check rarely restarts; 0: stop on the recurrence
+                          residual, as the reference does */
     int restriction;   /* TH_RESTRICT_*: how the V-cycle restricts its residual */
 } th_solver_opts;
 
