@@ -228,9 +228,8 @@ __device__ __forceinline__ void restrict_cell(const Geo& f, const Ops<T>& o, con
 // equal to restrict_cell in exact arithmetic, with 11 instead of 21 bytes
 // per fine cell (x of one colour, gE, gN; no b, gU, cap).
 template <class T>
-__device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __restrict__ gE, const T* __restrict__ gN,
-                                                   const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb, int I,
-                                                   int J, int l) {
+__device__ __forceinline__ T restrict_sum_fast(const Geo& f, const T* __restrict__ gE, const T* __restrict__ gN,
+                                               const T* __restrict__ fx, int I, int J, int l) {
     const long base = (long)l * f.plane;
     const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
     T r[2];
@@ -249,7 +248,13 @@ __device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __rest
     T sum = 0;
     sum += r[0];
     if (two) sum += r[1];
-    cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+    return sum;
+}
+template <class T>
+__device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __restrict__ gE, const T* __restrict__ gN,
+                                                   const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb, int I,
+                                                   int J, int l) {
+    cb[(long)l * cg.plane + cg.pos(I, J)] = restrict_sum_fast<T>(f, gE, gN, fx, I, J, l);
 }
 
 // Block (B.x, B.y) covers kColsX x blockDim.y coarse cells of plane B.z.
@@ -468,6 +473,87 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc, const T* __restrict__ xc,
     State* st, const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
     sweep_typed_body<T, NZ, SW_PROLONG, true>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+}
+
+// restrict_sum_fast with the red children's faces from their types (each
+// lane its own row).
+template <class T>
+__device__ __forceinline__ T restrict_sum_fast_typed(const Geo& f, const Typed<T>& y, const T* __restrict__ fx, int I,
+                                                     int J, int l) {
+    const long base = (long)l * f.plane;
+    const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
+    T r[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const Col c = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
+        const TypeRow<T> w = y.row(y.ty[c.c0], l);
+        const T Gw = c.hw ? w.v[TF_W] : T(0), Ge = c.he ? w.v[TF_E] : T(0);
+        const T Gs = c.hs ? w.v[TF_S] : T(0), Gn = c.hn ? w.v[TF_N] : T(0);
+        const T xw = fx[base + c.pw], xe = fx[base + c.pe], xs = fx[base + c.ps], xn = fx[base + c.pn];
+        T s = 0;  // absent faces add (+0) * x, as in sweep_column_typed
+        s += Gw * xw;
+        s += Ge * xe;
+        s += Gs * xs;
+        s += Gn * xn;
+        r[k] = s;
+    }
+    T sum = 0;
+    sum += r[0];
+    if (two) sum += r[1];
+    return sum;
+}
+
+// The lateral restriction of level f into a plane-parallel level g and g's
+// first (red, x = 0) sweep in one launch: lane (column of either colour,
+// plane) forms its coarse b (restrict_sum_fast's sum, FT: f's faces from its
+// column types) and stores it; the red columns then solve as sweep_pp_body
+// does in SW_ZERO, whose right-hand side is b itself. One launch and one
+// pass over the coarse b less per level; results identical.
+template <class T, int LP, bool FT>
+__global__ __launch_bounds__(kPPThreads) void k_restrict_zero_pp(Geo f, Ops<T> fo, Typed<T> fy,
+                                                                 const T* __restrict__ fx, Geo g, Ops<T> o,
+                                                                 const T* __restrict__ finv, T* __restrict__ b,
+                                                                 T* __restrict__ x, const State* __restrict__ st) {
+    constexpr int CPB = kPPThreads / LP, RS = CPB + 1;
+    __shared__ __align__(16) T sh_s[LP * RS], sh_i[LP * RS], sh_u[LP * RS];
+    __shared__ int sh_ok[kPPThreads / 4];
+    if (st->done) return;
+    const int tid = threadIdx.x, l = tid % LP, cb = tid / LP;
+    const long half = (long)g.hx * g.ny;             // columns of one colour
+    const long qq = (long)blockIdx.x * CPB + cb;     // red columns, then black
+    const int colour = qq >= half ? 1 : 0;
+    const long q = qq - colour * half;
+    const int j = (int)(q / g.hx), t = (int)(q - (long)j * g.hx);
+    const int i = 2 * t + ((j + colour) & 1);
+    const bool colok = qq < 2 * half && j < g.ny && i < g.nx;
+    const bool ok = colok && l < g.nz;
+    if (ok) {
+        const T sum = FT ? restrict_sum_fast_typed<T>(f, fy, fx, i, j, l)
+                         : restrict_sum_fast<T>(f, fo.gE, fo.gN, fx, i, j, l);
+        const long id = (long)l * g.plane + g.pos(i, j);
+        b[id] = sum;
+        if (colour == 0) {
+            sh_s[l * RS + cb] = sum;
+            sh_i[l * RS + cb] = finv[id];
+            sh_u[l * RS + cb] = o.gU[id];
+        }
+    }
+    if (l == 0) sh_ok[cb] = colok && colour == 0;
+    __syncthreads();
+    if (tid < CPB && sh_ok[tid]) {
+        T sv[LP], inv[LP], gu[LP];
+#pragma unroll
+        for (int k = 0; k < LP; ++k) {
+            sv[k] = sh_s[k * RS + tid];
+            inv[k] = sh_i[k * RS + tid];
+            gu[k] = sh_u[k * RS + tid];
+        }
+        column_thomas<T, LP>(g.nz, sv, inv, gu);
+#pragma unroll
+        for (int k = 0; k < LP; ++k) sh_s[k * RS + tid] = sv[k];
+    }
+    __syncthreads();
+    if (ok && colour == 0) x[(long)l * g.plane + g.pos(i, j)] = sh_s[l * RS + cb];
 }
 
 // restrict_cell_fast with the red children's faces from their types.

@@ -493,6 +493,34 @@ struct Solver final : SolverBase {
         return TH_OK;
     }
 
+    // Level c (lateral restriction, plane-parallel, above the tail) gets its
+    // restricted b and its first sweep from one launch (k_restrict_zero_pp).
+    bool fuse_restrict_zero(size_t c) const {
+        const Geo& g = L[c].g;
+        return fast && c != tail_k0 && (long)g.hx * g.ny <= kPPCols && g.nz <= 16;
+    }
+    int launch_restrict_zero(size_t k) {
+        const DevLevel<T>& f = L[k];
+        const DevLevel<T>& c = L[k + 1];
+        const long cols2 = 2L * c.g.hx * c.g.ny;
+        const unsigned nb = (unsigned)((cols2 * pp_lanes(c.g.nz) + kPPThreads - 1) / kPPThreads);
+#define TH_RZ(LP)                                                                                                  \
+    do {                                                                                                           \
+        if (f.ntypes)                                                                                              \
+            k_restrict_zero_pp<T, LP, true><<<nb, kPPThreads, 0, s>>>(f.g, f.ops(), f.typed(), f.x, c.g, c.ops(),  \
+                                                                      c.finv, c.b, c.x, st);                       \
+        else                                                                                                       \
+            k_restrict_zero_pp<T, LP, false><<<nb, kPPThreads, 0, s>>>(f.g, f.ops(), f.typed(), f.x, c.g, c.ops(), \
+                                                                       c.finv, c.b, c.x, st);                      \
+    } while (0)
+        if (c.g.nz <= 4) TH_RZ(4);
+        else if (c.g.nz <= 8) TH_RZ(8);
+        else TH_RZ(16);
+#undef TH_RZ
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+
     int launch_restrict(size_t k) {
         const DevLevel<T>& f = L[k];
         const DevLevel<T>& c = L[k + 1];
@@ -513,13 +541,15 @@ struct Solver final : SolverBase {
     // red. Level k's x starts from zero. Levels from tail_k0 down run in the
     // single-workgroup tail kernel. `cs`: convergence test carried by the
     // V-cycle's first kernel (level 0 only).
-    int vcycle(size_t k, Cons cs = {nullptr, 0, CONS_NONE}) {
+    // zeroed: level k's first sweep was run by the restriction into it.
+    int vcycle(size_t k, Cons cs = {nullptr, 0, CONS_NONE}, bool zeroed = false) {
         int rc;
         if (k == tail_k0) return launch_tail(cs);
-        if ((rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
+        if (!zeroed && (rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
         if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
-        if ((rc = launch_restrict(k))) return rc;
-        if ((rc = vcycle(k + 1))) return rc;
+        const bool fz = fuse_restrict_zero(k + 1);
+        if ((rc = fz ? launch_restrict_zero(k) : launch_restrict(k))) return rc;
+        if ((rc = vcycle(k + 1, Cons{nullptr, 0, CONS_NONE}, fz))) return rc;
         if ((rc = launch_sweep<SW_PROLONG>(k, 1))) return rc;
         return launch_sweep<SW_PLAIN>(k, 0);
     }
