@@ -603,12 +603,6 @@ template <class T>
 struct Tail {
     int n;  // levels in the tail; L[n-1] is the single column
     TailLevel<T> L[kMaxTail];
-    // rp = 1 (LDS tail): L[0]'s b is the lateral restriction of the level
-    // above it (geometry pg, faces pgE, pgN, x px: per-cell arrays), formed
-    // while the tail stages its levels rather than by a launch of its own.
-    int rp;
-    Geo pg;
-    const T *pgE, *pgN, *px;
 };
 
 template <class T, int NZ, int MODE, int TT>
@@ -795,16 +789,7 @@ __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const
             D.gN[d] = G.o.gN[q];
             D.gU[d] = G.o.gU[q];
             D.fi[d] = G.finv[q];
-            if (k == 0 && !t.rp) D.b[d] = G.b[q];
-        }
-    }
-    if (t.rp) {  // b of the top level, restricted from the level above (restrict_cell_fast's sum)
-        const LdsLevel<T> D = lds_level<T>(t, sm, 0, LP);
-        const Geo& g = t.L[0].g;
-        const int cells = g.nx * g.ny * g.nz;
-        for (int q = threadIdx.x; q < cells; q += kTailLdsThreads) {
-            const int I = q % g.nx, rest = q / g.nx, J = rest % g.ny, l = rest / g.ny;
-            D.b[(long)l * D.g.plane + g.pos(I, J)] = restrict_sum_fast<T>(t.pg, t.pgE, t.pgN, t.px, I, J, l);
+            if (k == 0) D.b[d] = G.b[q];
         }
     }
     __syncthreads();

@@ -467,14 +467,6 @@ struct Solver final : SolverBase {
             const DevLevel<T>& d = L[tail_k0 + k];
             t.L[k] = TailLevel<T>{d.g, d.ops(), d.finv, d.x, d.b};
         }
-        if (tail_restricts(tail_k0)) {
-            const DevLevel<T>& p = L[tail_k0 - 1];
-            t.rp = 1;
-            t.pg = p.g;
-            t.pgE = p.gE;
-            t.pgN = p.gN;
-            t.px = p.x;
-        }
         const int nz = g0.nz;
         if (lds_tail) {
 #define TH_TAIL(NZ) k_tail_lds<T, NZ><<<1, kTailLdsThreads, lds_bytes, s>>>(t, st, cs.part, cs.nblk, cs.mode)
@@ -507,9 +499,6 @@ struct Solver final : SolverBase {
         const Geo& g = L[c].g;
         return fast && c != tail_k0 && (long)g.hx * g.ny <= kPPCols && g.nz <= 16;
     }
-    // The LDS tail forms its top level's b from the level above it itself
-    // (Tail::rp) when that level keeps per-cell arrays.
-    bool tail_restricts(size_t c) const { return lds_tail && fast && c == tail_k0 && c > 0 && !L[c - 1].ntypes; }
     int launch_restrict_zero(size_t k) {
         const DevLevel<T>& f = L[k];
         const DevLevel<T>& c = L[k + 1];
@@ -559,7 +548,7 @@ struct Solver final : SolverBase {
         if (!zeroed && (rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
         if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
         const bool fz = fuse_restrict_zero(k + 1);
-        if (!tail_restricts(k + 1) && (rc = fz ? launch_restrict_zero(k) : launch_restrict(k))) return rc;
+        if ((rc = fz ? launch_restrict_zero(k) : launch_restrict(k))) return rc;
         if ((rc = vcycle(k + 1, Cons{nullptr, 0, CONS_NONE}, fz))) return rc;
         if ((rc = launch_sweep<SW_PROLONG>(k, 1))) return rc;
         return launch_sweep<SW_PLAIN>(k, 0);
