@@ -179,3 +179,29 @@ def test_true_residual_large_levels(nz, x0):
     xs = s.get_x()
     assert O.true_relres_flux(a, xs) <= 1.5e-10
     assert abs(s.true_relres() - O.true_relres_flux(a, xs)) <= 1e-6 * s.true_relres()
+
+
+@pytest.mark.parametrize("nz", [6, 16])
+def test_transient_large_levels_vs_oracle(nz):
+    """Backward-Euler steps on a grid with column types (the typed apply,
+    sweeps and residual with their C/dt terms, the start residual from the
+    previous step's x) against the oracle, step by step."""
+    layers = STACKS[nz] if nz != 16 else [SI + (4, 1), BOND + (2, 0), SI + (4, 1), TIM + (2, 0), CU + (4, 0)]
+    m = T.Model.from_layers(layers, 301, 171)
+    m.add_uniform_power(0, 20.0)
+    m.add_hotspot(0, 3e-3, 4e-3, 1e-3, 3.0)
+    m.add_tsv_region(0, 6e-3, 2e-3, 2e-3, 2e-3, 0.3)
+    m.finalize()
+    a = m.arrays()
+    kw = dict(dt0=1e-5, growth=1.5, dt_max=1e30, t_end=1e30, max_steps=8, tol=1e-9)
+    ref = O.run_transient(a, flexible=True, **kw)
+    s = T.Solver(m)
+    assert s.column_types() > 0
+    g = T.run_transient(s, **kw)
+    assert g["converged"] and len(g["t"]) == 8
+    assert np.array_equal(g["t"], ref["t"]) and np.array_equal(g["dt"], ref["dt"])
+    assert np.abs(g["iters"] - ref["iters"]).max() <= 1
+    assert np.abs(g["T"] - ref["T"]).max() < 1e-7 * ref["T"].max()
+    P = m.total_power()
+    eb = np.abs(g["q_out"] + g["q_stored"] - P) / P
+    assert eb.max() < 1e-6
