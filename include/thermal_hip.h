@@ -280,6 +280,11 @@ TH_API int th_surrogate_predict(th_surrogate* s, const float* in, int batch, flo
 /* One image; the output stays on the device: float [H][W][16], the first 10
  * channels used, valid until the next call on s. */
 TH_API int th_surrogate_predict_device(th_surrogate* s, const float* in, const float** d_out);
+/* The same with the input already in device memory (d_in: float [H][W][in
+ * channels], the layout of `in`; writes to it must be complete). No
+ * counterpart in the reference: for callers that keep their power maps on
+ * the device between solves, so the upload is not paid per prediction. */
+TH_API int th_surrogate_predict_device_input(th_surrogate* s, const float* d_in, const float** d_out);
 /* Loss (mean squared error over the 10 outputs) and its gradient with
  * respect to the flat parameters (grad may be NULL). */
 TH_API int th_surrogate_grad(th_surrogate* s, const float* in, const float* tgt, int batch, double* loss,

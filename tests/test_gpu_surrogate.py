@@ -233,3 +233,24 @@ def test_lift_relaxation_on_typed_grid():
     _line_relax_np(a, n, n, m.nz, ref, 0)
     _line_relax_np(a, n, n, m.nz, ref, 1)
     assert np.array_equal(relaxed, ref), np.abs(relaxed - ref).max()
+
+
+def test_predict_device_input_matches_host_input():
+    """th_surrogate_predict_device_input (input already in device memory)
+    gives the output of th_surrogate_predict_device bit for bit."""
+    n = 64
+    sur = T.Surrogate(n, n, seed=11)
+    x = np.random.default_rng(4).standard_normal((n, n, T.Surrogate.N_IN)).astype(np.float32)
+    import ctypes as C
+
+    def fetch(ptr):
+        out = np.empty((n, n, 16), np.float32)
+        hip = C.CDLL("libamdhip64.so.7")
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes), 2) == 0
+        return out
+
+    a = fetch(sur.predict_device(x))
+    d = T.DeviceBuffer(x)
+    b = fetch(sur.predict_device_input(d.ptr))
+    assert np.array_equal(a, b)
+    sur.close()
