@@ -422,10 +422,15 @@ struct Solver final : SolverBase {
         }
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
         if (d.ntypes) {
-            // FP32 prolongation: its own kernel (three coarse loads, occupancy floor)
+            // FP32 prolongation: its own kernels (three coarse loads; up to 16
+            // planes with the type-independent loads first, past that with an
+            // occupancy floor)
 #define TH_SWEEP(NZ)                                                                                                  \
     do {                                                                                                              \
-        if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value)                                            \
+        if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value && NZ <= 16)                                \
+            k_sweep_typed_prolong_lf<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x,   \
+                                                                         st, cs.part, cs.nblk, cs.mode);              \
+        else if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value)                                       \
             k_sweep_typed_prolong3<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st, \
                                                                        cs.part, cs.nblk, cs.mode);                    \
         else                                                                                                          \
