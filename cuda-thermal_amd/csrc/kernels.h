@@ -180,6 +180,28 @@ struct Vec2<double> {
     typedef double2 type;
 };
 
+// Loads with the nontemporal hint (global_load ... nt): their lines are the
+// first the caches give up. For streams a kernel reads once and no kernel
+// soon after reads again; the caches keep the arrays the next kernels reuse,
+// and fewer dirty lines are evicted early. Same values as plain loads.
+template <class T>
+struct NtVec2 {
+    typedef T type __attribute__((ext_vector_type(2)));
+};
+template <class T>
+__device__ __forceinline__ typename Vec2<T>::type load2_nt(const T* __restrict__ a, long k) {
+    typedef typename NtVec2<T>::type V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(a) + k);
+    typename Vec2<T>::type r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+template <class T>
+__device__ __forceinline__ T load_nt(const T* __restrict__ a, long k) {
+    return __builtin_nontemporal_load(a + k);
+}
+
 // ---- reductions -------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {
