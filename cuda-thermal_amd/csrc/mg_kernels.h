@@ -475,14 +475,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     sweep_typed_body<T, NZ, SW_PROLONG, true>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
-// The FP32 prolongation sweep (NZ <= 16) with every load that does not
-// depend on the column's type (b, the neighbours' x, the three coarse
+// FP32 plain and prolongation sweeps (NZ <= 16) with every load that does
+// not depend on the column's type (b, the neighbours' x, the three coarse
 // parents) issued before the wave's type test, so no wave waits for its
 // 2-byte type before them; then sweep_column_typed's arithmetic, in its
-// order, from the loaded values. At 1024^2 x 16: 31.3 -> 29.1 us warm and
-// 51.4 -> 48.5 us with the caches flushed, against k_sweep_typed_prolong3.
-// (The plain sweep gains only cold, FP64 and the zero sweep lose; past
-// NZ = 16 the five register columns would spill.)
+// order, from the loaded values. At 1024^2 x 16, prolongation: 31.3 -> 29.1
+// us warm and 51.4 -> 48.5 us with the caches flushed (against
+// k_sweep_typed_prolong3); plain: 36.9 -> 34.4 us flushed but 21.3 -> 22.4
+// warm, so only the pre-smoothing plain sweep, which runs on inputs the
+// caches do not hold, uses it. (FP64 and the zero sweep lose; past NZ = 16
+// the five register columns would spill.)
 template <class T, int NZ, bool UNI>
 __device__ __forceinline__ void sweep_column_typed_loaded(const Geo& g, const Typed<T>& y, T* __restrict__ x,
                                                           const Col& c, int t, const T (&bv)[NZ], const T (&xw)[NZ],
@@ -509,13 +511,14 @@ __device__ __forceinline__ void sweep_column_typed_loaded(const Geo& g, const Ty
     for (int l = 0; l < NZ; ++l)
         if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
 }
-template <class T, int NZ>
-__global__ __launch_bounds__(256) void k_sweep_typed_prolong_lf(Geo g, Typed<T> y, const T* __restrict__ b,
-                                                                T* __restrict__ x, int colour, Geo gc,
-                                                                const T* __restrict__ xc, State* st,
-                                                                const double* __restrict__ cons_part, int cons_nblk,
-                                                                int cons_mode) {
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_sweep_typed_lf(Geo g, Typed<T> y, const T* __restrict__ b,
+                                                        T* __restrict__ x, int colour, Geo gc,
+                                                        const T* __restrict__ xc, State* st,
+                                                        const double* __restrict__ cons_part, int cons_nblk,
+                                                        int cons_mode) {
     static_assert(NZ <= 16, "five register columns of NZ values");
+    static_assert(MODE == SW_PLAIN || MODE == SW_PROLONG, "plain or prolongation sweep");
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
     const int tt = blockIdx.x * kColsX + threadIdx.x;
@@ -526,7 +529,7 @@ __global__ __launch_bounds__(256) void k_sweep_typed_prolong_lf(Geo g, Typed<T> 
     const Col c = make_col(g, i, j);
     const int ty = y.ty[c.c0];
     const int nz = g.nz;
-    const Parents3 p3 = parents3(gc, c, i, j);
+    const Parents3 p3 = MODE == SW_PROLONG ? parents3(gc, c, i, j) : Parents3{0, 0, 0, false, false, false, false};
     T bv[NZ], xw[NZ], xe[NZ], xs[NZ], xn[NZ];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) {
@@ -537,12 +540,14 @@ __global__ __launch_bounds__(256) void k_sweep_typed_prolong_lf(Geo g, Typed<T> 
         xe[l] = x[base + c.pe];
         xs[l] = x[base + c.ps];
         xn[l] = x[base + c.pn];
-        const long cb = (long)lc * gc.plane;
-        const T c0 = xc[cb + p3.p0], cx = xc[cb + p3.px], cy = xc[cb + p3.py];
-        xw[l] = xw[l] + (p3.wx ? cx : c0);
-        xe[l] = xe[l] + (p3.ex ? cx : c0);
-        xs[l] = xs[l] + (p3.sy ? cy : c0);
-        xn[l] = xn[l] + (p3.nn ? cy : c0);
+        if (MODE == SW_PROLONG) {
+            const long cb = (long)lc * gc.plane;
+            const T c0 = xc[cb + p3.p0], cx = xc[cb + p3.px], cy = xc[cb + p3.py];
+            xw[l] = xw[l] + (p3.wx ? cx : c0);
+            xe[l] = xe[l] + (p3.ex ? cx : c0);
+            xs[l] = xs[l] + (p3.sy ? cy : c0);
+            xn[l] = xn[l] + (p3.nn ? cy : c0);
+        }
     }
     const int rep = wave_rep_type(ty);
     if (__all(ty == rep))

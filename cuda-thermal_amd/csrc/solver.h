@@ -403,7 +403,8 @@ struct Solver final : SolverBase {
 
     // Levels with at most kPPCols columns per colour run the plane-parallel
     // sweep (k_sweep_pp); larger ones the column walk (k_sweep).
-    template <int MODE>
+    // PRE: the pre-smoothing plain sweep (its inputs are not in the caches).
+    template <int MODE, bool PRE = false>
     int launch_sweep(size_t k, int colour, Cons cs = {nullptr, 0, CONS_NONE}) {
         const DevLevel<T>& d = L[k];
         const DevLevel<T>& c = L[std::min(k + 1, L.size() - 1)];
@@ -422,14 +423,15 @@ struct Solver final : SolverBase {
         }
         const dim3 grid = col_grid(d.g.hx, d.g.ny);
         if (d.ntypes) {
-            // FP32 prolongation: its own kernels (three coarse loads; up to 16
-            // planes with the type-independent loads first, past that with an
-            // occupancy floor)
+            // FP32 prolongation and pre-smoothing plain sweeps up to 16 planes:
+            // the type-independent loads first; FP32 prolongation past that:
+            // three coarse loads and an occupancy floor
 #define TH_SWEEP(NZ)                                                                                                  \
     do {                                                                                                              \
-        if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value && NZ <= 16)                                \
-            k_sweep_typed_prolong_lf<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x,   \
-                                                                         st, cs.part, cs.nblk, cs.mode);              \
+        if constexpr ((MODE == SW_PROLONG || (MODE == SW_PLAIN && PRE)) && std::is_same<T, float>::value &&           \
+                      NZ <= 16)                                                                                       \
+            k_sweep_typed_lf<T, NZ, MODE><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x,     \
+                                                                       st, cs.part, cs.nblk, cs.mode);                \
         else if constexpr (MODE == SW_PROLONG && std::is_same<T, float>::value)                                       \
             k_sweep_typed_prolong3<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.typed(), d.b, d.x, colour, c.g, c.x, st, \
                                                                        cs.part, cs.nblk, cs.mode);                    \
@@ -551,7 +553,7 @@ struct Solver final : SolverBase {
         int rc;
         if (k == tail_k0) return launch_tail(cs);
         if (!zeroed && (rc = launch_sweep<SW_ZERO>(k, 0, cs))) return rc;
-        if ((rc = launch_sweep<SW_PLAIN>(k, 1))) return rc;
+        if ((rc = launch_sweep<SW_PLAIN, true>(k, 1))) return rc;
         const bool fz = fuse_restrict_zero(k + 1);
         if ((rc = fz ? launch_restrict_zero(k) : launch_restrict(k))) return rc;
         if ((rc = vcycle(k + 1, Cons{nullptr, 0, CONS_NONE}, fz))) return rc;
