@@ -180,10 +180,11 @@ struct Vec2<double> {
     typedef double2 type;
 };
 
-// Loads with the nontemporal hint (global_load ... nt): their lines are the
-// first the caches give up. For streams a kernel reads once and no kernel
-// soon after reads again; the caches keep the arrays the next kernels reuse,
-// and fewer dirty lines are evicted early. Same values as plain loads.
+// Loads and stores with the nontemporal hint (global_load / global_store ...
+// nt): their lines are the first the caches give up. For streams that no
+// kernel soon after reads again; the caches keep the arrays the next kernels
+// reuse, and fewer dirty lines are evicted early. Same values as plain
+// loads and stores.
 template <class T>
 struct NtVec2 {
     typedef T type __attribute__((ext_vector_type(2)));
@@ -200,6 +201,14 @@ __device__ __forceinline__ typename Vec2<T>::type load2_nt(const T* __restrict__
 template <class T>
 __device__ __forceinline__ T load_nt(const T* __restrict__ a, long k) {
     return __builtin_nontemporal_load(a + k);
+}
+template <class T>
+__device__ __forceinline__ void store2_nt(T* __restrict__ a, long k, const typename Vec2<T>::type& v) {
+    typedef typename NtVec2<T>::type V;
+    V w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<V*>(a) + k);
 }
 
 // ---- reductions -------------------------------------------------------------

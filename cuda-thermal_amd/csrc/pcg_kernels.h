@@ -380,9 +380,11 @@ __device__ __forceinline__ void update_body(const Blk& B, long n, const double* 
         // Two cells per lane (16-byte loads of r and Ap): 98 against 105 us
         // at 1024^2 x 16. A level's planes hold an even number of slots, so n
         // is even; an odd last cell would go to lane 0 of block 0. Ap, r and
-        // z_old are read with the nontemporal hint (the V-cycle next reads b,
-        // not them): 97 -> 86 us back to back, 110 -> 99 us after a cache
-        // flush, at 1024^2 x 16.
+        // z_old are read, and r is written, with the nontemporal hint (the
+        // V-cycle next reads b, not them; r is next read after it). The loads:
+        // 97 -> 86 us back to back, 110 -> 99 us after a cache flush, at
+        // 1024^2 x 16; the r store (with the p update's x store) took the
+        // whole mixed solve from 9.0 to 8.7 ms.
         typedef typename Vec2<T>::type T2;
         const long n2 = n >> 1;
         for (long k = k0; k < n2; k += stride) {
@@ -391,7 +393,7 @@ __device__ __forceinline__ void update_body(const Blk& B, long n, const double* 
             double2 ri;
             ri.x = ro.x - alpha * a.x;
             ri.y = ro.y - alpha * a.y;
-            reinterpret_cast<double2*>(r)[k] = ri;
+            store2_nt(r, k, ri);
             v[0] += ri.x * ri.x;
             v[0] += ri.y * ri.y;
             if (MODE == UP_MG_FLEX) {
@@ -481,11 +483,13 @@ __device__ __forceinline__ void pupdate_body(const Blk& B, long n, const Z* __re
         const long k0 = (long)B.x * blockDim.x + threadIdx.x, n2 = n >> 1;
         for (long k = k0; k < n2; k += stride) {
             const P2 po = reinterpret_cast<const P2*>(p)[k];
+            // z and x with the nontemporal hint (no kernel reads them again
+            // before the next p update); p, which the apply reads next, not
             const Z2 zz = load2_nt(z, k);
             double2 xx = load2_nt(x, k);
             xx.x += alpha * (double)po.x;
             xx.y += alpha * (double)po.y;
-            reinterpret_cast<double2*>(x)[k] = xx;
+            store2_nt(x, k, xx);
             P2 pn;
             pn.x = (PT)((double)zz.x + beta * (double)po.x);
             pn.y = (PT)((double)zz.y + beta * (double)po.y);
