@@ -337,6 +337,26 @@ class Model:
         T = np.ascontiguousarray(T, np.float64)
         return lib().th_model_heat_out(self.h, _d(T))
 
+    def blocks(self):
+        """Power blocks of the case the model was built from (CaseSpec,
+        case3d.h:24-33): (name, layer, x0, y0, w, h, watts) each."""
+        out = []
+        for k in range(lib().th_model_block_count(self.h)):
+            nm = C.c_char_p()
+            layer = C.c_int()
+            v = [C.c_double() for _ in range(5)]
+            _check(lib().th_model_block_info(self.h, k, C.byref(nm), C.byref(layer), *[C.byref(x) for x in v]))
+            out.append((nm.value.decode(), layer.value) + tuple(x.value for x in v))
+        return out
+
+    def block_temps(self, k, T):
+        """Max and area-weighted mean rise of T over block k (block_temps,
+        main3d.cu:286-301)."""
+        T = np.ascontiguousarray(T, np.float64)
+        tmax, tavg = C.c_double(), C.c_double()
+        _check(lib().th_model_block_temps(self.h, k, _d(T), C.byref(tmax), C.byref(tavg)))
+        return tmax.value, tavg.value
+
     def desc(self):
         d = ModelDesc()
         _check(lib().th_model_describe(self.h, C.byref(d)))
