@@ -895,12 +895,20 @@ __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const
         tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s, scr_c);
         tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s, scr_c);
     }
+    // Only slots of real columns come back: a padding slot (odd nx: the
+    // half-row's last slot in the rows whose colour has no column there) is
+    // never written in LDS, so it holds whatever an earlier workgroup left
+    // there, and it must stay 0 in global memory: the PCG's r.z and
+    // r.z_old run over every slot, and r = 0 there times an Inf or NaN left
+    // in LDS would make them NaN.
     const LdsLevel<T> T0 = lds_level<T>(t, sm, 0, LP);
     const Geo G0 = t.L[0].g;
     const long cells0 = G0.cells();
     for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) {
         const long l = q / G0.plane, sl = q - l * G0.plane;
-        t.L[0].x[q] = sl < G0.slots() ? T0.x[l * T0.g.plane + sl] : T(0);
+        const int row = (int)(sl / G0.hx), ts = (int)(sl - (long)row * G0.hx);
+        const bool real = sl < G0.slots() && 2 * ts + (((row >> 1) + (row & 1)) & 1) < G0.nx;
+        t.L[0].x[q] = real ? T0.x[l * T0.g.plane + sl] : T(0);
     }
 }
 template <class T, int NZ>
