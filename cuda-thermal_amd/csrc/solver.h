@@ -103,6 +103,11 @@ struct SolverBase {
     virtual int column_types() const = 0;
     virtual int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
                          int* hot_cell) = 0;
+    virtual int check_colorize_args(int l0, int l1, float rise_hi) const = 0;
+    virtual long colorize_blocks() const = 0;
+    virtual int colorize_into(int l0, int l1, float rise_hi, unsigned* dst, unsigned long long* part,
+                              unsigned long long* key, unsigned char* host_rgba, float* hot_rise, int* hot_cell) = 0;
+    virtual int device() const = 0;
 };
 
 template <class T>
@@ -961,34 +966,51 @@ struct Solver final : SolverBase {
     unsigned long long *map_part = nullptr, *map_key = nullptr;
     int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
                  int* hot_cell) override {
-        if (l0 < 0 || l1 < l0 || l1 >= g0.nz) return fail(TH_ERR_ARG, "colorize: planes out of range");
-        if (!(rise_hi > 0.f) || !std::isfinite(rise_hi)) return fail(TH_ERR_ARG, "colorize: rise_hi must be > 0");
-        const dim3 grid((unsigned)((g0.nx + 63) / 64), (unsigned)((g0.ny + 3) / 4));
-        const int nb = (int)(grid.x * grid.y);
         int rc;
-        if (!map_part && ((rc = dalloc(&map_part, nb)) || (rc = dalloc(&map_key, 1)))) return rc;
+        if ((rc = check_colorize(l0, l1, rise_hi))) return rc;
+        if (!map_part && ((rc = dalloc(&map_part, colorize_blocks())) || (rc = dalloc(&map_key, 1)))) return rc;
         unsigned* dst = nullptr;
         if (rgba && device_rgba) dst = (unsigned*)rgba;
         else if (rgba) {
             if (!map_rgba && (rc = dalloc(&map_rgba, (size_t)g0.nx * g0.ny))) return rc;
             dst = map_rgba;
         }
-        k_colorize<<<grid, 256, 0, s>>>(g0, x, l0, l1, rise_hi, dst, map_part);
+        return colorize_into(l0, l1, rise_hi, dst, map_part, map_key, rgba && !device_rgba ? rgba : nullptr, hot_rise,
+                             hot_cell);
+    }
+    int check_colorize(int l0, int l1, float rise_hi) const {
+        if (l0 < 0 || l1 < l0 || l1 >= g0.nz) return fail(TH_ERR_ARG, "colorize: planes out of range");
+        if (!(rise_hi > 0.f) || !std::isfinite(rise_hi)) return fail(TH_ERR_ARG, "colorize: rise_hi must be > 0");
+        return TH_OK;
+    }
+    dim3 colorize_grid() const { return dim3((unsigned)((g0.nx + 63) / 64), (unsigned)((g0.ny + 3) / 4)); }
+    long colorize_blocks() const override {
+        const dim3 g = colorize_grid();
+        return (long)g.x * g.y;
+    }
+    // k_colorize into dst (device image or null) with one partial per block
+    // in part and the key in key, both device memory of the caller's; the
+    // host image (host_rgba, from dst) and the hottest cell to the host.
+    int colorize_into(int l0, int l1, float rise_hi, unsigned* dst, unsigned long long* part, unsigned long long* key,
+                      unsigned char* host_rgba, float* hot_rise, int* hot_cell) override {
+        const dim3 grid = colorize_grid();
+        k_colorize<<<grid, 256, 0, s>>>(g0, x, l0, l1, rise_hi, dst, part);
         HIPTRY(hipGetLastError());
-        k_max_key<<<1, 256, 0, s>>>(map_part, nb, map_key);
+        k_max_key<<<1, 256, 0, s>>>(part, (int)colorize_blocks(), key);
         HIPTRY(hipGetLastError());
-        unsigned long long key = 0;
-        HIPTRY(hipMemcpyAsync(&key, map_key, sizeof(key), hipMemcpyDeviceToHost, s));
-        if (rgba && !device_rgba)
-            HIPTRY(hipMemcpyAsync(rgba, map_rgba, (size_t)g0.nx * g0.ny * 4, hipMemcpyDeviceToHost, s));
+        unsigned long long kv = 0;
+        HIPTRY(hipMemcpyAsync(&kv, key, sizeof(kv), hipMemcpyDeviceToHost, s));
+        if (host_rgba) HIPTRY(hipMemcpyAsync(host_rgba, dst, (size_t)g0.nx * g0.ny * 4, hipMemcpyDeviceToHost, s));
         HIPTRY(hipStreamSynchronize(s));
-        const unsigned bits = (unsigned)(key >> 32);
+        const unsigned bits = (unsigned)(kv >> 32);
         float v;
         std::memcpy(&v, &bits, 4);
         if (hot_rise) *hot_rise = v;
-        if (hot_cell) *hot_cell = (int)(key & 0xffffffffu);
+        if (hot_cell) *hot_cell = (int)(kv & 0xffffffffu);
         return TH_OK;
     }
+    int check_colorize_args(int l0, int l1, float rise_hi) const override { return check_colorize(l0, l1, rise_hi); }
+    int device() const override { return opt.device; }
     // ||P - A v|| / ||P|| of the steady operator (flux form)
     int relres_of(const double* v, double* out) {
         int rc;

@@ -2,6 +2,39 @@
 // th::Solver (solver.h).
 #include "solver.h"
 
+#include <mutex>
+
+// Scratch of th_solver_colorize_device_noalloc: the die map's per-block
+// partial keys and the final key, in a file-scope device array instead of
+// memory allocated at the first call. Every GPU the library runs on holds its
+// own copy (136 KB) until the process ends. Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): each device's address comes from hipGetSymbolAddress, taken once
+// with that device current.
+namespace {
+constexpr long kNoAllocBlocks = 16384;  // 64 x 4 columns each: grids of up to 2048 x 2048 columns
+constexpr int kNoAllocDevices = 64;
+__device__ __align__(256) unsigned long long scratch[kNoAllocBlocks + 1];
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kNoAllocDevices] = {};
+
+// The device address of `scratch` on `dev` (current), looked up on the first
+// call for that device.
+int scratch_on(int dev, unsigned long long** out) {
+    if (dev < 0 || dev >= kNoAllocDevices) return th::fail(TH_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return th::fail(TH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned long long*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TH_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI: solver
 // ===========================================================================
@@ -128,6 +161,29 @@ int th_solver_colorize_device(th_solver* s, int l0, int l1, float rise_hi, unsig
     TH_S(s);
     if (!d_rgba) return fail(TH_ERR_ARG, "null device image");
     return s->impl->colorize(l0, l1, rise_hi, d_rgba, true, hot_rise, hot_cell);
+}
+int th_solver_colorize_device_noalloc(th_solver* s, int l0, int l1, float rise_hi, unsigned char* d_rgba,
+                                      float* hot_rise, int* hot_cell) {
+    TH_S(s);
+    int rc = s->impl->check_colorize_args(l0, l1, rise_hi);
+    if (rc) return rc;
+    const long nb = s->impl->colorize_blocks();
+    if (nb > kNoAllocBlocks)
+        return fail(TH_ERR_ARG, "colorize_device_noalloc: grid of " + std::to_string(nb) +
+                                    " blocks of 64 x 4 columns exceeds the static scratch (" +
+                                    std::to_string(kNoAllocBlocks) + " blocks: at most 2048 x 2048 columns)");
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) return fail(TH_ERR_HIP, "hipGetDevice failed");
+    const int dev = s->impl->device();
+    if (dev != prev && hipSetDevice(dev) != hipSuccess) return fail(TH_ERR_HIP, "hipSetDevice failed");
+    unsigned long long* sc = nullptr;
+    rc = scratch_on(dev, &sc);
+    // Every partial slot is written before k_max_key reads it, and the key
+    // slot is written by k_max_key: nothing in the scratch has to start at 0.
+    if (!rc) rc = s->impl->colorize_into(l0, l1, rise_hi, (unsigned*)d_rgba, sc, sc + kNoAllocBlocks, nullptr,
+                                         hot_rise, hot_cell);
+    if (dev != prev) (void)hipSetDevice(prev);
+    return rc;
 }
 int th_solver_precondition(th_solver* s, const double* r, double* z) {
     TH_S(s);

@@ -126,6 +126,7 @@ SIGNATURES = {
     "th_solver_column_types": (C.c_int, [_vp]),
     "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
+    "th_solver_colorize_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_precondition": (C.c_int, [_vp, _dp, _dp]),
     "th_solver_apply_operator": (C.c_int, [_vp, _dp, _dp]),
     "th_solver_time_kernel": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),

@@ -184,6 +184,18 @@ TH_API int th_solver_colorize(th_solver* s, int l0, int l1, float rise_hi, unsig
  * bytes, e.g. a buffer a display maps), as the viewer writes its PBO. */
 TH_API int th_solver_colorize_device(th_solver* s, int l0, int l1, float rise_hi, unsigned char* d_rgba,
                                      float* hot_rise, int* hot_cell);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under graph capture, tight display or serving loops,
+ * harnesses that forbid hipMalloc): th_solver_colorize_device without any
+ * allocation. Its per-block partials and hottest-cell key live in a static
+ * device array of the library (136 KB per GPU, held until the process ends)
+ * instead of memory allocated on the first call. Limit: grids of at most
+ * 16384 blocks of 64 x 4 columns (2048 x 2048 columns); larger grids fail
+ * with TH_ERR_ARG. d_rgba may be NULL (hottest cell only). Calls on one GPU
+ * share that array: calls that overlap in time on one GPU (other threads,
+ * other solvers' streams) race and must be serialised by the caller. */
+TH_API int th_solver_colorize_device_noalloc(th_solver* s, int l0, int l1, float rise_hi, unsigned char* d_rgba,
+                                             float* hot_rise, int* hot_cell);
 /* z = M^-1 r: one preconditioner application (the multigrid V-cycle in the
  * solver's precision, or Jacobi) for the current dt, host arrays in the
  * natural layout (CpuPcg::precondition, pcg3d_cpu.cpp:115-124). */
