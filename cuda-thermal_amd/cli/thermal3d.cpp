@@ -6,15 +6,18 @@
 //   thermal3d transient [n] [jacobi]   heat-up after power-on; writes transient.csv
 //   thermal3d case <dir> [n]           solve a case folder; writes tmap_layer<k>.csv
 //   thermal3d profile [n]              one timed multigrid solve
+//   thermal3d export3dice <dir> <cell_um> <outdir>   write the case for 3D-ICE
+//   thermal3d compare3dice <dir> <cell_um> <icedir>  compare with 3D-ICE's results
 //
 // Library errors print the message and exit(1), as the reference's
 // CUDA_CHECK does (cuda_check.h:8-16). The CPU path of the reference is not
 // part of this library, so where the reference's verify compares CPU with
 // GPU, this one compares the GPU's multigrid and Jacobi solves, the exact 1D
 // chain and the conservation checks (the CPU comparison lives in the test
-// suite). 3D-ICE export/compare needs the external 3D-ICE tool and is not
-// provided.
+// suite). export3dice writes the 3D-ICE input files and compare3dice reads
+// the Tmap / Tflp files 3D-ICE wrote for them; neither runs 3D-ICE itself.
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -529,13 +532,138 @@ int run_case(const std::string& dir, int n) {
     return 0;
 }
 
+// export_case_3dice (main3d.cu:405-416)
+int export_case_3dice(const std::string& dir, double cell_um, const std::string& outdir) {
+    int nx = 0, ny = 0;
+    if (th_case_export_3dice(dir.c_str(), cell_um, outdir.c_str(), &nx, &ny) != TH_OK) {
+        std::printf("Export failed: %s\n", th_last_error());
+        return 1;
+    }
+    std::printf("Wrote %s/case.stk and floorplans (%g um cells, %d x %d).\n", outdir.c_str(), cell_um, nx, ny);
+    std::printf("Run 3D-ICE there:  3D-ICE-Emulator case.stk\n");
+    return 0;
+}
+
+// compare_case_3dice (main3d.cu:418-488): the case solved on the grid of the
+// export, each die's top plane against 3D-ICE's Tmap (K, row y = 0 first)
+// and each block's max / average against its Tflp files.
+int compare_case_3dice(const std::string& dir, double cell_um, const std::string& icedir) {
+    th_model* probe = nullptr;
+    if (th_model_load_case(dir.c_str(), 1, 1, 1, &probe) != TH_OK) {
+        std::printf("Could not load case '%s': %s\n", dir.c_str(), th_last_error());
+        return 1;
+    }
+    double Lx, Ly, ht, hb, amb;
+    CK(th_model_settings(probe, &Lx, &Ly, &ht, &hb, &amb));
+    th_model_destroy(probe);
+    const int nx = (int)std::lround(Lx * 1e6 / cell_um), ny = (int)std::lround(Ly * 1e6 / cell_um);
+    if (nx < 1 || ny < 1) {
+        std::printf("Could not load case '%s': cell size %g um gives no cells\n", dir.c_str(), cell_um);
+        return 1;
+    }
+    th_model* h = nullptr;
+    CK(th_model_load_case(dir.c_str(), nx, ny, 1, &h));
+    Model m(h);
+    Result g = solve(m, 1e-10, TH_PRECOND_MG);
+    std::printf("\n== compare with 3D-ICE: %s, %d x %d x %d cells, %.1f W ==\n", dir.c_str(), nx, ny, m.nz, m.power());
+    std::printf("  ours: %d multigrid iterations, %.1f ms (+ %.0f ms setup), true residual %.1e\n", g.iters,
+                g.solve_ms, g.setup_ms, true_relres(m, g.T));
+    std::vector<int> pl(m.nz);
+    CK(th_model_plane_layer(m.h, pl.data()));
+    int failures = 0;
+    for (int L = 0; L < th_model_layer_count(m.h); ++L) {
+        double lt, lk, lrc;
+        int lp, active;
+        const char* lname = nullptr;
+        CK(th_model_layer_info(m.h, L, &lt, &lk, &lrc, &lp, &active, &lname));
+        if (!active) continue;
+        int plane = -1;  // the layer's top plane
+        for (int l = 0; l < m.nz; ++l)
+            if (pl[l] == L) plane = l;
+        const std::string path = icedir + "/tmap_L" + std::to_string(L) + ".txt";
+        int rows = 0, cols = 0;
+        std::vector<double> ice;
+        bool ok = th_3dice_read_tmap(path.c_str(), &rows, &cols, nullptr, 0) == TH_OK && rows == ny && cols == nx;
+        if (ok) {
+            ice.resize((size_t)rows * cols);
+            ok = th_3dice_read_tmap(path.c_str(), &rows, &cols, ice.data(), (long)ice.size()) == TH_OK;
+        }
+        if (!ok) {
+            std::printf("  %s: missing, or not %d rows x %d columns\n", path.c_str(), ny, nx);
+            ++failures;
+            continue;
+        }
+        double maxd = 0, sumd = 0, ours_max = -1e30, ice_max = -1e30;
+        int wi = 0, wj = 0;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double a = g.T[((size_t)plane * ny + j) * nx + i] + amb, b = ice[(size_t)j * nx + i] - 273.15;
+                const double d = a - b;
+                sumd += d;
+                if (std::fabs(d) > maxd) {
+                    maxd = std::fabs(d);
+                    wi = i;
+                    wj = j;
+                }
+                ours_max = std::max(ours_max, a);
+                ice_max = std::max(ice_max, b);
+            }
+        std::printf("\n  %s (whole map, %d cells):\n", lname, nx * ny);
+        std::printf("    hottest cell: ours %.3f C, 3D-ICE %.3f C\n", ours_max, ice_max);
+        std::printf("    max |difference| = %.4f C at x = %.2f mm, y = %.2f mm; mean difference %+.4f C\n", maxd,
+                    (wi + 0.5) * m.dx * 1e3, (wj + 0.5) * m.dy * 1e3, sumd / (nx * ny));
+        // Tflp files: block names and one value each
+        auto read_tflp = [](const std::string& p, std::vector<std::string>& names, std::vector<double>& vals) {
+            int n = 0;
+            if (th_3dice_read_tflp(p.c_str(), nullptr, 0, nullptr, 0, &n) != TH_OK) return false;
+            const int len = 256;
+            std::vector<char> buf((size_t)n * len);
+            vals.resize(n);
+            if (th_3dice_read_tflp(p.c_str(), buf.data(), len, vals.data(), n, &n) != TH_OK) return false;
+            names.clear();
+            for (int k = 0; k < n; ++k) names.emplace_back(buf.data() + (size_t)k * len);
+            return true;
+        };
+        std::vector<std::string> nmax, navg;
+        std::vector<double> vmax, vavg;
+        if (!read_tflp(icedir + "/tflp_max_L" + std::to_string(L) + ".txt", nmax, vmax) ||
+            !read_tflp(icedir + "/tflp_avg_L" + std::to_string(L) + ".txt", navg, vavg)) {
+            std::printf("    (no Tflp files)\n");
+            continue;
+        }
+        std::printf("    %-14s %10s %10s %8s %10s %10s %8s\n", "block", "max ours", "max 3DICE", "diff", "avg ours",
+                    "avg 3DICE", "diff");
+        for (int k = 0; k < th_model_block_count(m.h); ++k) {
+            const char* bname = nullptr;
+            int layer;
+            double x0, y0, w, hh, W;
+            CK(th_model_block_info(m.h, k, &bname, &layer, &x0, &y0, &w, &hh, &W));
+            if (layer != L) continue;
+            std::string id(bname);  // 3D-ICE element names: letters and digits, '_' for the rest
+            for (char& c : id)
+                if (!std::isalnum((unsigned char)c)) c = '_';
+            const size_t a = std::find(nmax.begin(), nmax.end(), id) - nmax.begin();
+            const size_t b = std::find(navg.begin(), navg.end(), id) - navg.begin();
+            if (a >= vmax.size() || b >= vavg.size()) continue;
+            double tmax, tavg;
+            CK(th_model_block_temps(m.h, k, g.T.data(), &tmax, &tavg));
+            tmax += amb;
+            tavg += amb;
+            const double imax = vmax[a] - 273.15, iavg = vavg[b] - 273.15;
+            std::printf("    %-14s %10.3f %10.3f %+8.4f %10.3f %10.3f %+8.4f\n", bname, tmax, imax, tmax - imax, tavg,
+                        iavg, tavg - iavg);
+        }
+    }
+    return failures ? 1 : 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc >= 2 && (std::strcmp(argv[1], "export3dice") == 0 || std::strcmp(argv[1], "compare3dice") == 0)) {
-        std::printf("%s: needs the external 3D-ICE tool, which is not bundled\n", argv[1]);
-        return 1;
-    }
+    if (argc >= 5 && std::strcmp(argv[1], "export3dice") == 0)
+        return export_case_3dice(argv[2], std::atof(argv[3]), argv[4]);
+    if (argc >= 5 && std::strcmp(argv[1], "compare3dice") == 0)
+        return compare_case_3dice(argv[2], std::atof(argv[3]), argv[4]);
     if (argc >= 3 && std::strcmp(argv[1], "case") == 0) return run_case(argv[2], argc >= 4 ? std::atoi(argv[3]) : 200);
     if (argc >= 2 && std::strcmp(argv[1], "profile") == 0) {
         const int n = argc >= 3 ? std::atoi(argv[2]) : 800;

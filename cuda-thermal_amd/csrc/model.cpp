@@ -81,12 +81,6 @@ std::vector<Layer> default_layers() {
     };
 }
 
-// TSV effective medium with copper fraction f: vertical paths in parallel,
-// lateral by the Rayleigh / Maxwell-Garnett rule for parallel cylinders.
-static inline double kz_mix(double k, double kc, double f) { return f * kc + (1 - f) * k; }
-static inline double kxy_mix(double k, double kc, double f) {
-    return k * ((1 + f) * kc + (1 - f) * k) / ((1 - f) * kc + (1 + f) * k);
-}
 // Conductance of two half-cells in series: area / (d1/(2 k1) + d2/(2 k2)).
 static inline double series(double area, double d1, double k1, double d2, double k2) {
     return area / (d1 / (2 * k1) + d2 / (2 * k2));

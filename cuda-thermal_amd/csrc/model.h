@@ -43,6 +43,14 @@ struct Model {
     size_t at(int i, int j, int l) const { return ((size_t)l * ny + j) * nx + i; }
 };
 
+// TSV effective medium with copper fraction f: vertical paths in parallel,
+// lateral by the Rayleigh / Maxwell-Garnett rule for parallel cylinders
+// (stack3d.h:20-25).
+inline double kz_mix(double k, double kc, double f) { return f * kc + (1 - f) * k; }
+inline double kxy_mix(double k, double kc, double f) {
+    return k * ((1 + f) * kc + (1 - f) * k) / ((1 - f) * kc + (1 + f) * k);
+}
+
 int make_model(const std::vector<Layer>& layers, int nx, int ny, double Lx, double Ly,
                double h_top, double h_bot, Model& m);
 std::vector<Layer> default_layers();
@@ -73,6 +81,10 @@ struct Case {
     std::vector<Tsv> tsvs;
 };
 bool read_case(const std::string& dir, Case& c, std::string& err);
+// 3D-ICE interchange (ice.cpp)
+int export_3dice(const Case& c, double cell_um, const std::string& outdir, int* nx, int* ny);
+int read_tmap(const std::string& path, std::vector<double>& vals, int& rows, int& cols);
+int read_tflp(const std::string& path, std::vector<std::string>& names, std::vector<double>& vals);
 Case builtin_example();
 int case_model(const Case& c, int nx, int ny, bool with_tsv, Model& m);
 

@@ -108,6 +108,9 @@ SIGNATURES = {
     "th_model_tsv_count": (C.c_int, [_vp]),
     "th_model_block_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), _ip, _dp, _dp, _dp, _dp, _dp]),
     "th_model_block_temps": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "th_case_export_3dice": (C.c_int, [C.c_char_p, C.c_double, C.c_char_p, _ip, _ip]),
+    "th_3dice_read_tmap": (C.c_int, [C.c_char_p, _ip, _ip, _dp, C.c_long]),
+    "th_3dice_read_tflp": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, _dp, C.c_int, _ip]),
     "th_solver_default_opts": (None, [C.POINTER(SolverOpts)]),
     "th_solver_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(SolverOpts), C.POINTER(_vp)]),
     "th_solver_destroy": (None, [_vp]),
@@ -362,6 +365,34 @@ class Model:
         d = ModelDesc()
         _check(lib().th_model_describe(self.h, C.byref(d)))
         return d
+
+
+def export_3dice(folder, cell_um, outdir):
+    """Writes case folder `folder` as a 3D-ICE stack (case.stk, floorplans,
+    TSV layouts) on cells of cell_um micrometres into outdir (export_3dice,
+    case3d.h:224-383); returns the grid (nx, ny)."""
+    nx, ny = C.c_int(), C.c_int()
+    _check(lib().th_case_export_3dice(str(folder).encode(), cell_um, str(outdir).encode(), C.byref(nx), C.byref(ny)))
+    return nx.value, ny.value
+
+
+def read_3dice_tmap(path):
+    """A 3D-ICE Tmap file as a (rows, cols) array of K."""
+    rows, cols = C.c_int(), C.c_int()
+    _check(lib().th_3dice_read_tmap(str(path).encode(), C.byref(rows), C.byref(cols), None, 0))
+    out = np.empty(rows.value * cols.value, np.float64)
+    _check(lib().th_3dice_read_tmap(str(path).encode(), C.byref(rows), C.byref(cols), _d(out), out.size))
+    return out.reshape(rows.value, cols.value)
+
+
+def read_3dice_tflp(path):
+    """A 3D-ICE Tflp file as {block name: K}, in file order."""
+    n = C.c_int()
+    _check(lib().th_3dice_read_tflp(str(path).encode(), None, 0, None, 0, C.byref(n)))
+    names = C.create_string_buffer(256 * n.value)
+    vals = np.empty(n.value, np.float64)
+    _check(lib().th_3dice_read_tflp(str(path).encode(), names, 256, _d(vals), n.value, C.byref(n)))
+    return {names.raw[256 * k:256 * (k + 1)].split(b"\0", 1)[0].decode(): float(vals[k]) for k in range(n.value)}
 
 
 def default_model(n, ny=None):

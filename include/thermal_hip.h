@@ -90,6 +90,29 @@ TH_API int th_model_block_info(const th_model* m, int k, const char** name, int*
  * on its layer's planes (block_temps, main3d.cu:286-301). */
 TH_API int th_model_block_temps(const th_model* m, int k, const double* T, double* tmax, double* tavg);
 
+/* ---- 3D-ICE interchange: case3d.h:192-400, main3d.cu:380-416 ------------
+ * export_case_3dice / export_3dice: the case folder `dir` described for
+ * 3D-ICE on square cells of cell_um micrometres, written into the existing
+ * folder outdir: case.stk, L<k>.flp per active layer (die) and LT<k>_<p>.lay
+ * per plane with TSVs; the same text, byte for byte, as the reference's
+ * exporter. Refused (TH_ERR_CASE, with the reference's message) when the
+ * cell size does not divide the die, an active layer has TSVs or more than
+ * one plane, there is a bottom heat sink, or two power blocks of one layer
+ * overlap. nx, ny (may be NULL): the lateral grid used. */
+TH_API int th_case_export_3dice(const char* dir, double cell_um, const char* outdir, int* nx, int* ny);
+/* read_3dice_tmap (case3d.h:387-400): a Tmap result file, rows of values
+ * (K) with '%' comment lines; *rows x *cols values, row-major, into values
+ * (capacity doubles; values may be NULL to ask for the size). Rows of
+ * different lengths are an error (TH_ERR_CASE). */
+TH_API int th_3dice_read_tmap(const char* path, int* rows, int* cols, double* values, long capacity);
+/* read_3dice_tflp (main3d.cu:380-403): a Tflp result file, the block names
+ * of its "% Time(s) name(K) ..." header and one value (K) per block. Names
+ * go to names + k * name_len (NUL-terminated), values to values[k], for up
+ * to capacity blocks; *count receives the number of blocks (names and
+ * values may both be NULL to ask for it). */
+TH_API int th_3dice_read_tflp(const char* path, char* names, int name_len, double* values, int capacity,
+                              int* count);
+
 /* What the solver reads from a finalized Model3D (pcg3d_gpu.cu:372-391).
  * All arrays are copied by th_solver_create. */
 typedef struct {

@@ -402,8 +402,18 @@ def ref():
         L.ref_precondition.argtypes = [_vp, C.c_double, _dp, _dp]
         L.ref_run_transient.restype = C.c_int
         L.ref_run_transient.argtypes = [_vp, C.c_int] + [C.c_double] * 5 + [C.c_int, _dp, C.c_int, _dp, _dp]
+        L.ref_export_3dice.restype = C.c_int
+        L.ref_export_3dice.argtypes = [C.c_char_p, C.c_double, C.c_char_p, C.c_char_p, C.c_int]
         _ref = L
     return _ref
+
+
+def ref_export_3dice(folder, cell_um, outdir):
+    """The reference's 3D-ICE exporter (case3d.h:224-383) on a case folder:
+    None on success, else the exporter's error message."""
+    err = C.create_string_buffer(512)
+    rc = ref().ref_export_3dice(str(folder).encode(), cell_um, str(outdir).encode(), err, 512)
+    return None if rc == 0 else err.value.decode()
 
 
 class RefModel:

@@ -55,6 +55,18 @@ void* ref_model_case(const char* dir, int nx, int ny, int with_tsv, char* err, i
     }
     return new Model3D(build_case_model(cs, nx, ny, with_tsv != 0));
 }
+// The reference's 3D-ICE exporter (case3d.h:224-383) on a case folder: 0 on
+// success, else 1 with the exporter's message copied into err.
+int ref_export_3dice(const char* dir, double cell_um, const char* outdir, char* err, int errlen) {
+    CaseSpec cs;
+    std::string e;
+    if (!load_case(dir, cs, e) || !export_3dice(cs, cell_um, outdir, e)) {
+        std::strncpy(err, e.c_str(), errlen - 1);
+        err[errlen - 1] = 0;
+        return 1;
+    }
+    return 0;
+}
 void* ref_model_example_case(int nx, int ny, int with_tsv) {
     return new Model3D(build_case_model(example_case(), nx, ny, with_tsv != 0));
 }

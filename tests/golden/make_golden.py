@@ -1,12 +1,15 @@
 """Writes the golden fixtures of tests/golden/ from the reference's own CPU
 path (oracle/_ref/libref.so, compiled from the reference sources by
 `make -C oracle ref`). Run in the build container, where the reference is
-mounted:  python tests/golden/make_golden.py
+mounted:  python tests/golden/make_golden.py [solver] [ice]
+(the .npz fixtures of the solver path, the 3D-ICE exports; both by default)
 
 Each fixture holds inputs and the reference's outputs only (arrays in .npz,
 scalars in the same file). The reference itself never travels.
 """
+import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -38,9 +41,85 @@ def hotspot_ref(n, ny=None, at=(3e-3, 3e-3)):
     return r.finalize()
 
 
-def main():
+ICE = os.path.join(HERE, "ice")  # 3D-ICE exports of the reference's exporter
+
+
+def ice_inputs():
+    """Case folders for the export checks: the reference's own validation
+    cases, and variants of them that reach each of the exporter's branches
+    (a die with no blocks, each refusal). Written under ice/inputs/."""
+    base = os.path.join(CASES, "validate_3dice")
+    out = {}
+
+    def variant(name, stack_edit=None, floor_edit=None, tsv=None):
+        d = os.path.join(ICE, "inputs", name)
+        os.makedirs(d, exist_ok=True)
+        for f in ("stack.csv", "floorplan.csv"):
+            text = open(os.path.join(base, f)).read()
+            if f == "stack.csv" and stack_edit:
+                text = stack_edit(text)
+            if f == "floorplan.csv" and floor_edit:
+                text = floor_edit(text)
+            open(os.path.join(d, f), "w").write(text)
+        if tsv:
+            open(os.path.join(d, "tsv.csv"), "w").write(tsv)
+        out[name] = d
+
+    # tier 1 without blocks: its floorplan is the single empty element
+    variant("die_without_blocks", floor_edit=lambda t: "\n".join(l for l in t.splitlines() if "tier 1" not in l) + "\n")
+    variant("overlapping_blocks", floor_edit=lambda t: t + "extra, Si tier 2 (bottom), 1.5, 1.5, 2.0, 2.0, 1\n")
+    variant("bottom_sink", stack_edit=lambda t: t.replace("# h_bottom, 0", "# h_bottom, 500"))
+    variant("warm_ambient", stack_edit=lambda t: t.replace("# h_top, 2e4", "# h_top, 1.25e4").replace(
+        "# ambient_C, 25", "# ambient_C, 40.5"))
+    variant("two_plane_die", stack_edit=lambda t: t.replace("Si tier 2 (bottom), 50, 130, 1.66e6, 1, 1",
+                                                            "Si tier 2 (bottom), 50, 130, 1.66e6, 2, 1"))
+    variant("tsv_in_die", tsv="layer, x_mm, y_mm, w_mm, h_mm, cu_fraction\nSi tier 1 (top), 0.5, 0.5, 2.0, 2.0, 0.15\n")
+    variant("tsv_stack", tsv="layer, x_mm, y_mm, w_mm, h_mm, cu_fraction\n"
+                             "base, 1.03, 2.01, 3.3, 0.77, 0.4\nbond + BEOL, 0.5, 0.5, 2.0, 2.0, 0.15\n"
+                             "bond + BEOL, 1.5, 1.5, 2.0, 2.0, 0.2\nCu spreader, 7.0, 7.0, 1.0, 2.5, 0.3\n")
+    return out
+
+
+def ice_fixtures():
+    """The reference exporter's files for the cases that export, and its
+    message for each it refuses (ice/exports.json lists both)."""
+    if os.path.isdir(ICE):
+        shutil.rmtree(ICE)
+    os.makedirs(ICE)
+    inputs = ice_inputs()
+    runs = [("validate_tsv", os.path.join(CASES, "validate_tsv"), 100), ("validate_tsv", os.path.join(CASES, "validate_tsv"), 50),
+            ("validate_3dice", os.path.join(CASES, "validate_3dice"), 100),
+            ("validate_3dice", os.path.join(CASES, "validate_3dice"), 250),
+            ("die_without_blocks", inputs["die_without_blocks"], 200), ("tsv_stack", inputs["tsv_stack"], 100),
+            ("example", os.path.join(CASES, "example"), 100), ("tsv_under_core0", os.path.join(CASES, "tsv_under_core0"), 100),
+            ("validate_3dice", os.path.join(CASES, "validate_3dice"), 300),
+            ("overlapping_blocks", inputs["overlapping_blocks"], 100), ("bottom_sink", inputs["bottom_sink"], 100),
+            ("warm_ambient", inputs["warm_ambient"], 125),
+            ("two_plane_die", inputs["two_plane_die"], 100), ("tsv_in_die", inputs["tsv_in_die"], 100)]
+    errors = {}
+    for name, folder, cell in runs:
+        out = os.path.join(ICE, f"{name}_{cell}um")
+        os.makedirs(out)
+        err = O.ref_export_3dice(folder, cell, out)
+        rel = os.path.relpath(folder, HERE)
+        if err is not None:
+            os.rmdir(out)
+            errors[f"{name}_{cell}um"] = dict(case=rel, cell_um=cell, error=err)
+        else:
+            errors[f"{name}_{cell}um"] = dict(case=rel, cell_um=cell, error=None,
+                                              files=sorted(os.listdir(out)))
+        print(f"3D-ICE export {name} at {cell} um: {err or 'ok'}")
+    with open(os.path.join(ICE, "exports.json"), "w") as f:
+        json.dump(errors, f, indent=1, sort_keys=True)
+
+
+def main(sections):
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` first")
+    if "ice" in sections:
+        ice_fixtures()
+    if "solver" not in sections:
+        return
 
     # 1. Model arrays: hotspot stack on an odd grid; example case with TSVs;
     #    a case folder loaded by the reference's loader.
@@ -103,4 +182,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # sections: "solver" (the .npz fixtures) and "ice" (3D-ICE exports); both by default
+    main(sys.argv[1:] or ["solver", "ice"])

@@ -57,8 +57,9 @@ def test_cli_reports_a_bad_case_folder(tmp_path):
     cli = os.path.join(ROOT, "cuda-thermal_amd", "bin", "thermal3d")
     p = subprocess.run([cli, "case", str(tmp_path / "missing"), "16"], capture_output=True, text=True, timeout=60)
     assert p.returncode == 1 and "Could not load case" in p.stdout and "stack.csv" in p.stdout
-    p = subprocess.run([cli, "export3dice", "a", "1", "b"], capture_output=True, text=True, timeout=60)
-    assert p.returncode == 1
+    p = subprocess.run([cli, "export3dice", str(tmp_path / "missing"), "100", str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 1 and p.stdout.startswith("Export failed: cannot open") and "stack.csv" in p.stdout
 
 
 def test_case_blocks_exported():
