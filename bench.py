@@ -83,63 +83,81 @@ def cpu_info():
     return dict(model=model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)))
 
 
-def cpu_baseline(folder, n, iters, transient_iters=887):
-    """The reference's CPU MG-PCG (oracle/_ref, built from the reference
-    sources) or else the oracle's restatement, single-threaded and pinned to
-    one core, on the same inputs: config 2 (this case folder) as a bounded
-    sample of 2 PCG iterations scaled to a solve of `iters` iterations plus
-    the start; config 0 (128x128x10 hotspot stack) solved in full; config 1
-    (800x800x10, 100 backward-Euler steps) estimated from a 2-iteration
-    sample and the transient's iteration count."""
+def cpu_baseline(model, transient_iters=887):
+    """The CPU path, single-threaded and pinned to one core, on the same
+    inputs: the oracle's restatement of the reference's CpuPcg (the
+    reference's plain FP64 algorithm: FP64 V-cycle, Fletcher-Reeves beta,
+    residual-form restriction; pcg3d_cpu.cpp:115-185 with -DMG_DOUBLE), a full
+    solve of the 1024^2 x 16 stack to rel-res 1e-10 (its field is also the
+    FP64 oracle every GPU mode is compared with); config 0 (128x128x10
+    hotspot stack) solved in full; config 1 (800x800x10, 100 backward-Euler
+    steps) estimated from a 2-iteration sample and the transient's
+    iteration count."""
     import oracle as O
 
-    sample_it = 2
     info = cpu_info()
     old = os.sched_getaffinity(0)
     core = min(old)
     os.sched_setaffinity(0, {core})  # this thread: the CPU solves run in it
     try:
+        a = model.arrays()
+        s = O.Solver(a, "mg", mg_double=True)
+        s.set_x_zero()
+        s.set_rhs_steady()
         t0 = time.perf_counter()
-        if O.ref_available():
-            kind = "reference"
-            m = O.RefModel.case(folder, n)
-            solve_ms = m.cpu_pcg(TOL, maxit=sample_it, precond="mg")["solve_ms"]
-        else:
-            kind = "port"
-            import thermal_amd as TT
+        st = s.solve(TOL, 500)
+        solve_ms = (time.perf_counter() - t0) * 1e3
+        T_ref = s.get_x()
+        del s
 
-            a = TT.Model.load_case(folder, n).arrays()
-            s = O.Solver(a, "mg")
-            s.set_x_zero()
-            s.set_rhs_steady()
+        def hotspot_solve(nn, maxit):
+            h = T.hotspot_model(nn, 5e-3, 5e-3).arrays()
+            c = O.Solver(h, "mg", mg_double=True)
+            c.set_x_zero()
+            c.set_rhs_steady()
             t1 = time.perf_counter()
-            s.solve(TOL, sample_it)
-            solve_ms = (time.perf_counter() - t1) * 1e3
-        wall = time.perf_counter() - t0
-        per_it = solve_ms / (sample_it + 1)
-        extra = {}
-        if kind == "reference":
-            def hotspot(nn, at):
-                h = O.RefModel.default(nn)
-                h.add_uniform_power(0, 22.0)
-                h.add_hotspot(0, at, at, 1e-3, 3.0)
-                h.add_uniform_power(2, 25.0)
-                h.finalize()
-                return h
+            r = c.solve(TOL, maxit)
+            return r, (time.perf_counter() - t1) * 1e3
 
-            r0 = hotspot(128, 5e-3).cpu_pcg(TOL, maxit=500, precond="mg")
-            extra["config0_128x128x10"] = dict(ms=round(r0["solve_ms"], 1), iters=int(r0["iters"]))
-            r1 = hotspot(800, 5e-3).cpu_pcg(TOL, maxit=sample_it, precond="mg")
-            it1 = r1["solve_ms"] / (sample_it + 1)
-            extra["config1_800x800x10_100_steps_estimate_s"] = round(it1 * (transient_iters + 100) / 1e3, 1)
+        r0, ms0 = hotspot_solve(128, 500)
+        r1, ms1 = hotspot_solve(800, 2)
+        it1 = ms1 / (r1["iters"] + 1)
     finally:
         os.sched_setaffinity(0, old)
-    value = per_it * (iters + 1)
-    return dict(value=round(value, 1), unit="ms per solve (estimated)", cores=1, kind=kind,
-                sample=(f"{n}x{n}x16 config, {sample_it} MG-PCG iterations ({solve_ms:.0f} ms, {wall:.1f} s wall "
-                        f"incl. model and level setup) on 1 core (pinned to cpu {core}), scaled to {iters} "
-                        f"iterations + start"),
-                ms_per_iteration=round(per_it, 1), cpu=info, **extra)
+    out = dict(value=round(solve_ms, 1), unit="ms per solve", cores=1, kind="port",
+               sample=(f"one full FP64 MG-PCG solve of the same 1024x1024x16 stack to rel-res 1e-10 "
+                       f"({st['iters']} iterations, converged={st['converged']}) on 1 core (pinned to cpu {core}); "
+                       "the oracle's restatement of the reference CPU path"),
+               ms_per_iteration=round(solve_ms / (st["iters"] + 1), 1), iters=st["iters"], cpu=info,
+               config0_128x128x10=dict(ms=round(ms0, 1), iters=int(r0["iters"])),
+               config1_800x800x10_100_steps_estimate_s=round(it1 * (transient_iters + 100) / 1e3, 1))
+    return out, T_ref
+
+
+def solve_bytes_per_cell(solver, precision):
+    """Algorithmic HBM bytes of one PCG iteration per fine cell, and of the
+    solve's one-off work, for this build's kernels (each distinct array once
+    per kernel). Vector kernels on the fine level: apply 12 / 16, update
+    32 / 32, dot 12 / 16, p update 28 / 40 B per cell (FP32 / FP64 search
+    direction and V-cycle). Per V-cycle level, weighted by its share of the
+    fine cells: the zero, plain, prolongation and plain sweeps and the
+    lateral restriction out of it, 26 B per cell in FP32 with column types
+    (b and x only) and 74 without (the per-cell faces and 1/pivot too),
+    twice that in FP64; the tail's levels move nothing through HBM but their
+    staging, which is left out. Once per solve: the start residual (b = P,
+    x = 0), the replacement residual and the true-residual check, 16 + 40 +
+    40 B per cell. Returns (per iteration, once per solve)."""
+    sz = 4 if precision == "mixed" else 8
+    vec = (12 + 32 + 12 + 28) if precision == "mixed" else (16 + 32 + 16 + 40)
+    lv = solver.levels()
+    n0 = lv[0]["nx"] * lv[0]["ny"] * lv[0]["nz"]
+    vc = 0.0
+    for L in lv:
+        if L["kind"] == "tail":
+            continue
+        per = (26 if L["types"] else 74) * sz / 4
+        vc += per * L["nx"] * L["ny"] * L["nz"] / n0
+    return vec + vc, 16 + 40 + 40
 
 
 def transient_leg():
@@ -167,6 +185,48 @@ def transient_leg():
     out["max_abs_err_predictor_vs_fp64_K"] = float(np.abs(out.pop("mixed+predictor_T") - out["fp64_T"]).max())
     out["max_abs_err_mixed_vs_fp64_K"] = float(np.abs(out.pop("mixed_T") - out.pop("fp64_T")).max())
     out["workload"] = "800x800x10 two-tier stack, 25 W + 25 W with a 3 W hotspot at (5,5) mm, 100 BE steps"
+    return out
+
+
+def transient_config2_leg(model):
+    """The 1024^2 x 16 TSV stack's 100-step backward-Euler run from ambient
+    (dt0 = 10 us growing 10% per step, no cap, tol 1e-9), FP64 and mixed,
+    and mixed with the extrapolated step start: wall time, iterations, worst
+    per-step energy balance, final T against the FP64 run."""
+    P = model.total_power()
+    kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    out, fields = {}, {}
+    for mode in ("fp64", "mixed", "mixed+predictor"):
+        s = T.Solver(model, precision=mode.split("+")[0])
+        pred = mode.endswith("predictor")
+        T.run_transient(s, predictor=pred, **dict(kw, max_steps=3))  # warm-up
+        g = T.run_transient(s, predictor=pred, **kw)
+        out[mode] = dict(ms=round(g["solve_ms"], 2), steps=len(g["t"]), t_end_s=float(g["t"][-1]),
+                         iterations=int(g["iters"].sum()), converged=g["converged"],
+                         energy_balance_max=float((np.abs(g["q_out"] + g["q_stored"] - P) / P).max()),
+                         T_max_C=float(g["T_max"][-1]) + 25.0)
+        fields[mode] = g["T"]
+        s.close()
+        log(f"  transient 1024^2x16 {mode}: {out[mode]}")
+    out["max_abs_err_mixed_vs_fp64_K"] = float(np.abs(fields["mixed"] - fields["fp64"]).max())
+    out["max_abs_err_predictor_vs_fp64_K"] = float(np.abs(fields["mixed+predictor"] - fields["fp64"]).max())
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    out["workload"] = "1024x1024x16 TSV stack, 100 BE steps from ambient, dt0 10 us x 1.1^k"
+    return out
+
+
+def side_leg(model, label, steps=5):
+    """ms per steady solve to true rel-res 1e-10, FP64 and mixed, of another
+    stack (config 0, or a grid where the fine level keeps per-cell arrays)."""
+    out = {}
+    for prec, beta in (("fp64", "fr"), ("mixed", "flexible")):
+        s = T.Solver(model, precision=prec, beta=beta, true_residual=True)
+        ms, st = time_solves(s, steps, 2)
+        out[prec] = dict(ms=round(float(np.mean(ms)), 3), iters=st["iters"], converged=st["converged"],
+                         true_relres=s.true_relres(), column_types=s.column_types())
+        s.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    log(f"  {label}: {out}")
     return out
 
 
@@ -205,7 +265,7 @@ def learned_leg(model, spec, steps, warmup, x_ref):
     res = dict(ms=float(np.mean(ms)), ms_min=float(np.min(ms)), iters=st["iters"], converged=st["converged"],
                true_relres=s.true_relres(), energy_balance=abs(s.heat_out() - P) / P,
                max_abs_err_vs_fp64_K=float(np.abs(x - x_ref).max()), forward_ms=fwd_ms,
-               forward_tflops=fwd_flop / (fwd_ms * 1e-3) / 1e12, **info)
+               forward_tflops=fwd_flop / (fwd_ms * 1e-3) / 1e12, _x=x, **info)
     log(f"  mixed+learned: {res['ms']:.2f} ms/solve, {res['iters']} iterations, lifted guess used: {used} "
         f"(start rel-res {rl:.2e} lifted vs {rp:.2e} plain), forward {fwd_ms:.3f} ms")
     s.close()
@@ -277,6 +337,7 @@ def main():
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-side", action="store_true", help="skip the config-0 and per-cell-path legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (a single solve "
@@ -292,10 +353,16 @@ def main():
     log(f"  model: {model.n} cells, {model.total_power():.2f} W")
 
     s64, x64, r64 = run_mode(model, "fp64", args.steps, args.warmup, precision="fp64", beta="fr")
+    b64 = solve_bytes_per_cell(s64, "fp64")
     s64.close()
     smx, xmx, rmx = run_mode(model, "mixed", args.steps, args.warmup, precision="mixed", beta="flexible")
+    bmx = solve_bytes_per_cell(smx, "mixed")
     err_mixed = float(np.abs(xmx - x64).max())
     log(f"  max |mixed - fp64| = {err_mixed:.2e} K")
+    for r, (bi, bo) in ((r64, b64), (rmx, bmx)):
+        traffic = (r["iters"] * bi + bo) * model.n
+        r["hbm_bytes_per_iteration_per_cell"] = round(bi, 1)
+        r["hbm_frac_whole_solve"] = round(traffic / (r["ms"] * 1e-3) / (PEAK_GBS * 1e9), 4)
 
     # Kernels of the mixed iteration, timed back to back with HIP events on
     # the solver's stream. The dominant one (largest time per iteration: the
@@ -320,23 +387,42 @@ def main():
     if os.path.exists(cnt):
         with open(cnt) as f:
             traffic = json.load(f).get(dom, {}).get("traffic_bytes_per_launch")
+    smx.close()
 
     tr = None if args.no_transient else transient_leg()
+    tr2 = None if args.no_transient else transient_config2_leg(model)
     learned = None if args.no_learned else learned_leg(model, W.config2_spec(args.seed), args.steps, args.warmup, x64)
     batched = None if args.no_batched else batched_leg()
+    side = None
+    if not args.no_side:
+        side = dict(config0_128x128x10=side_leg(T.hotspot_model(128, 5e-3, 5e-3), "config 0, 128x128x10"),
+                    per_cell_path_1000x1000x16=side_leg(W.build(W.config2_spec(args.seed, fraction_seed=7), 1000),
+                                                        "per-cell path, 1000x1000x16 (columns do not repeat)"))
+        side["per_cell_path_1000x1000x16"]["workload"] = (
+            "the same floorplan on 1000x1000x16 with a copper fraction per TSV cluster drawn from U[0.05, 1]: "
+            "the clusters cover cells in part and no few column types describe the grid, so the fine level "
+            "reads per-cell faces")
 
     cpu = None
-    if not args.no_cpu:
-        log("  cpu baseline (bounded sample) ...")
-        cpu = cpu_baseline(folder, n, r64["iters"], tr["fp64"]["iterations"] if tr else 887)
-        log(f"  cpu: {cpu}")
-
-    head = learned if learned else rmx
-    head_mode = ("mixed+learned (surrogate warm start, FP32 V-cycle, FP64 flexible PCG)" if learned
-                 else "mixed (FP32 V-cycle, FP64 flexible PCG)")
     modes = {"fp64": r64, "mixed": rmx}
     if learned:
         modes["mixed+learned"] = learned
+    if not args.no_cpu:
+        log("  cpu baseline: full FP64 solve on one core (also the FP64 oracle field) ...")
+        cpu, T_oracle = cpu_baseline(model, tr["fp64"]["iterations"] if tr else 887)
+        log(f"  cpu: {cpu}")
+        fields = {"fp64": x64, "mixed": xmx}
+        if learned:
+            fields["mixed+learned"] = learned.pop("_x")
+        for k, x in fields.items():
+            modes[k]["max_abs_err_vs_fp64_oracle_K"] = float(np.abs(x - T_oracle).max())
+            log(f"  {k}: max |GPU - FP64 oracle| = {modes[k]['max_abs_err_vs_fp64_oracle_K']:.2e} K")
+    elif learned:
+        learned.pop("_x")
+
+    head = learned if learned else rmx
+    head_mode = ("mixed+learned (surrogate warm start with fallback, FP32 V-cycle, FP64 flexible PCG)" if learned
+                 else "mixed (FP32 V-cycle, FP64 flexible PCG)")
     line = {
         "metric": METRIC,
         "value": round(head["ms"], 3),
@@ -348,7 +434,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 PCG + f32 V-cycle",
+        "dtype": "f64 PCG (x, r, Ap, dots, alpha, beta, residual checks) + f32 V-cycle and f32 search direction p",
         "data": "synthetic (seeded floorplan and TSV clusters)",
         "config": {"workload": f"{n}x{n}x16 TSV stack, steady MG-PCG from T=0 to true rel-res 1e-10",
                    "seed": args.seed, "cells": model.n, "mode": head_mode},
@@ -360,8 +446,11 @@ def main():
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic, "kernel": names[dom],
                      "bytes_per_launch": kt[dom]["mb"] * 1e6, "avg_launch_us": kt[dom]["us"]},
         "kernels": kt,
+        "vcycle_us": round(vc_ms * 1e3, 1),
         "transient": tr,
+        "transient_1024x1024x16": tr2,
         "batched": batched,
+        "side": side,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -101,6 +101,7 @@ struct SolverBase {
                              int* used, double* rp, double* rl) = 0;
     virtual int nz() const = 0;
     virtual int column_types() const = 0;
+    virtual int level_info(int k, int* nx, int* ny, int* nz, int* ntypes, int* kind) const = 0;
     virtual int colorize(int l0, int l1, float rise_hi, unsigned char* rgba, bool device_rgba, float* hot_rise,
                          int* hot_cell) = 0;
     virtual int check_colorize_args(int l0, int l1, float rise_hi) const = 0;
@@ -957,6 +958,18 @@ struct Solver final : SolverBase {
     }
     int nz() const override { return g0.nz; }
     int column_types() const override { return L.empty() ? 0 : L[0].ntypes; }
+    // Level k's size, column types and which kernels run it: 0 the column
+    // walk, 1 the plane-parallel sweeps, 2 the single-workgroup tail.
+    int level_info(int k, int* nx, int* ny, int* nz, int* ntypes, int* kind) const override {
+        if (k < 0 || k >= nlevels()) return fail(TH_ERR_ARG, "level out of range");
+        const Geo& g = L[k].g;
+        if (nx) *nx = g.nx;
+        if (ny) *ny = g.ny;
+        if (nz) *nz = g.nz;
+        if (ntypes) *ntypes = L[k].ntypes;
+        if (kind) *kind = (size_t)k >= tail_k0 ? 2 : ((long)g.hx * g.ny <= kPPCols && g.nz <= 16 ? 1 : 0);
+        return TH_OK;
+    }
 
     int true_relres(double* out) override { return relres_of(x, out); }
 

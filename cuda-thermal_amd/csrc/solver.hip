@@ -151,6 +151,10 @@ int th_solver_heat_out(th_solver* s, double* out) {
 }
 int th_solver_nlevels(th_solver* s) { return s ? s->impl->nlevels() : -1; }
 int th_solver_column_types(th_solver* s) { return s ? s->impl->column_types() : -1; }
+int th_solver_level_info(th_solver* s, int k, int* nx, int* ny, int* nz, int* ntypes, int* kind) {
+    TH_S(s);
+    return s->impl->level_info(k, nx, ny, nz, ntypes, kind);
+}
 int th_solver_colorize(th_solver* s, int l0, int l1, float rise_hi, unsigned char* rgba, float* hot_rise,
                        int* hot_cell) {
     TH_S(s);

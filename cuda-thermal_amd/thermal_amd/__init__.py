@@ -127,6 +127,7 @@ SIGNATURES = {
     "th_solver_heat_out": (C.c_int, [_vp, _dp]),
     "th_solver_nlevels": (C.c_int, [_vp]),
     "th_solver_column_types": (C.c_int, [_vp]),
+    "th_solver_level_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip, _ip]),
     "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
@@ -501,6 +502,17 @@ class Solver:
     def column_types(self):
         """Distinct fine-level column types in use (0: per-cell arrays)."""
         return lib().th_solver_column_types(self.h)
+
+    def levels(self):
+        """Per multigrid level: dict(nx, ny, nz, types, kind), kind one of
+        "column walk", "plane-parallel", "tail"."""
+        out = []
+        for k in range(self.nlevels()):
+            v = [C.c_int() for _ in range(5)]
+            _check(lib().th_solver_level_info(self.h, k, *[C.byref(x) for x in v]))
+            out.append(dict(nx=v[0].value, ny=v[1].value, nz=v[2].value, types=v[3].value,
+                            kind=("column walk", "plane-parallel", "tail")[v[4].value]))
+        return out
 
     def colorize(self, l0, l1, rise_hi, image=True):
         """Die map of the current T (viewer.cu:68-87): the hottest rise of

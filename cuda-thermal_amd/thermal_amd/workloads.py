@@ -90,10 +90,13 @@ def tsv_clusters(seed, count, lattice=64):
     return [((q % lattice) * side, (q // lattice) * side, side, side) for q in sorted(picked)]
 
 
-def config2_spec(seed=1000, tsv=True, cluster_count=205, active=(True, True)):
+def config2_spec(seed=1000, tsv=True, cluster_count=205, active=(True, True), fraction_seed=None):
     """The 1024^2 x 16 TSV stack of the benchmark: per die 12 random blocks
     (seed + die) on a 10% uniform background, 25 W per die, and TSV clusters.
-    `active` switches a die's power off (both on by default)."""
+    `active` switches a die's power off (both on by default). fraction_seed:
+    each cluster's copper fraction drawn from U[0.05, 1] instead of 1 (on a
+    grid whose cells the clusters cover in part, no two columns then need be
+    alike)."""
     layers = config2_layers()
     die_layer = (0, 2)  # Si tier 2 (bottom), Si tier 1 (top)
     bond = 1
@@ -105,9 +108,11 @@ def config2_spec(seed=1000, tsv=True, cluster_count=205, active=(True, True)):
             blocks.append((f"die{d}_b{k}", die_layer[d]) + b)
         uniform.append((die_layer[d], 2.5))
         if tsv:
+            frac = SplitMix64(fraction_seed + d) if fraction_seed is not None else None
             for c in tsv_clusters(seed + 1000 + d, cluster_count):
-                tsvs.append((die_layer[d],) + c + (1.0,))
-                tsvs.append((bond,) + c + (1.0,))
+                f = frac.uniform(0.05, 1.0) if frac else 1.0
+                tsvs.append((die_layer[d],) + c + (f,))
+                tsvs.append((bond,) + c + (f,))
     return dict(layers=layers, blocks=blocks, uniform=uniform, tsvs=tsvs, h_top=2e4, h_bot=0.0, Lx=DIE, Ly=DIE)
 
 

@@ -194,6 +194,12 @@ TH_API int th_solver_nlevels(th_solver* s);
  * 2-byte type per column and a table of the types' faces instead of the
  * per-cell arrays, 0 when they read the arrays, -1 for a null solver. */
 TH_API int th_solver_column_types(th_solver* s);
+/* Multigrid level k (0 = fine, th_solver_nlevels - 1 = the single column;
+ * no reference counterpart; introspection): its grid, its number of column
+ * types (0: per-cell arrays) and which kernels run it (*kind: 0 the column
+ * walk, 1 the plane-parallel sweeps, 2 the single-workgroup coarse tail).
+ * Any output may be NULL. */
+TH_API int th_solver_level_info(th_solver* s, int k, int* nx, int* ny, int* nz, int* ntypes, int* kind);
 /* Die map of the current x (the viewer's k_colorize, viewer.cu:68-87): for
  * each column (i, j) the hottest temperature rise v over planes l0..l1 (at
  * least 0), coloured on [0, rise_hi] with the viewer's five-stop palette
