@@ -64,9 +64,10 @@ def run_mode(model, name, steps, warmup, **opts):
     P = model.total_power()
     res = dict(ms=float(np.mean(ms)), ms_min=float(np.min(ms)), iters=st["iters"], converged=st["converged"],
                true_relres=s.true_relres(), energy_balance=abs(s.heat_out() - P) / P, setup_ms=setup_ms,
-               T_max_C=float(x.max()) + 25.0)
+               setup_breakdown_ms={k: round(v, 1) for k, v in s.setup_times().items()}, T_max_C=float(x.max()) + 25.0)
     log(f"  {name:6s}: {res['ms']:.2f} ms/solve (min {res['ms_min']:.2f}), {res['iters']} iterations, "
-        f"true rel-res {res['true_relres']:.2e}, energy balance {res['energy_balance']:.1e}, setup {setup_ms:.0f} ms")
+        f"true rel-res {res['true_relres']:.2e}, energy balance {res['energy_balance']:.1e}, setup {setup_ms:.0f} ms "
+        f"{res['setup_breakdown_ms']}")
     return s, x, res
 
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 
 namespace th {
@@ -15,24 +16,52 @@ static std::vector<double> merge_pairs(const std::vector<double>& w) {
     return c;
 }
 
-static HostLevel coarsen(const HostLevel& f) {
+// Runs fn(k) for k = 0 .. n-1 on up to hardware_concurrency threads (at
+// most 16), each taking a contiguous range of k.
+template <class Fn>
+static void parallel_for(long n, Fn fn) {
+    const long hw = std::max(1u, std::thread::hardware_concurrency());
+    const long nt = std::max(1L, std::min({n, hw, 16L}));
+    if (nt == 1) {
+        for (long k = 0; k < n; ++k) fn(k);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (long t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            for (long k = n * t / nt; k < n * (t + 1) / nt; ++k) fn(k);
+        });
+    for (std::thread& th : pool) th.join();
+}
+
+// The fine arrays of one level, natural layout (not owned).
+struct LevelView {
+    int nx, ny, nz;
+    const std::vector<double>*wx, *wy;
+    const float *gE, *gN, *gU, *gB, *cap;
+};
+
+// Coarse operator of a level (mg3d.h:100-136), plane by plane in parallel;
+// every coarse value is one double sum in the children's natural order.
+static HostLevel coarsen(const LevelView& f) {
     HostLevel c;
     c.nx = (f.nx + 1) / 2;
     c.ny = (f.ny + 1) / 2;
     c.nz = f.nz;
-    c.wx = merge_pairs(f.wx);
-    c.wy = merge_pairs(f.wy);
+    c.wx = merge_pairs(*f.wx);
+    c.wy = merge_pairs(*f.wy);
     const size_t fp = (size_t)f.nx * f.ny, cp = (size_t)c.nx * c.ny, cn = cp * c.nz;
     c.gE.assign(cn, 0.0f);
     c.gN.assign(cn, 0.0f);
     c.gU.assign(cn, 0.0f);
     c.cap.assign(cn, 0.0f);
     c.gB.assign(cp, 0.0f);
-    for (int l = 0; l < c.nz; ++l) {
-        const float* fU = f.gU.data() + l * fp;
-        const float* fC = f.cap.data() + l * fp;
-        const float* fE = f.gE.data() + l * fp;
-        const float* fN = f.gN.data() + l * fp;
+    const std::vector<double>&fwx = *f.wx, &fwy = *f.wy;
+    parallel_for(c.nz, [&](long l) {
+        const float* fU = f.gU + l * fp;
+        const float* fC = f.cap + l * fp;
+        const float* fE = f.gE + l * fp;
+        const float* fN = f.gN + l * fp;
         for (int J = 0; J < c.ny; ++J) {
             const int j0 = 2 * J, j1 = std::min(2 * J + 1, f.ny - 1);
             for (int I = 0; I < c.nx; ++I) {
@@ -49,43 +78,44 @@ static HostLevel coarsen(const HostLevel& f) {
                 c.gU[cid] = (float)up;
                 c.cap[cid] = (float)heat;
                 if (l == 0) c.gB[(size_t)J * c.nx + I] = (float)bot;
-                if (I + 1 < c.nx) {  // fine faces between columns 2I+1 and 2I+2
-                    double s = 0;
-                    for (int j = j0; j <= j1; ++j) s += fE[(size_t)j * f.nx + 2 * I + 1];
-                    // scaled by fine / coarse centre distance across the face
-                    const double df = 0.5 * (f.wx[2 * I + 1] + f.wx[2 * I + 2]);
+                if (I + 1 < c.nx) {  // the fine faces between columns 2I+1 and 2I+2
+                    double sum = 0;
+                    for (int j = j0; j <= j1; ++j) sum += fE[(size_t)j * f.nx + 2 * I + 1];
+                    // times fine over coarse centre distance across the face
+                    const double df = 0.5 * (fwx[2 * I + 1] + fwx[2 * I + 2]);
                     const double dc = 0.5 * (c.wx[I] + c.wx[I + 1]);
-                    c.gE[cid] = (float)(s * df / dc);
+                    c.gE[cid] = (float)(sum * df / dc);
                 }
                 if (J + 1 < c.ny) {
-                    double s = 0;
-                    for (int i = i0; i <= i1; ++i) s += fN[(size_t)(2 * J + 1) * f.nx + i];
-                    const double df = 0.5 * (f.wy[2 * J + 1] + f.wy[2 * J + 2]);
+                    double sum = 0;
+                    for (int i = i0; i <= i1; ++i) sum += fN[(size_t)(2 * J + 1) * f.nx + i];
+                    const double df = 0.5 * (fwy[2 * J + 1] + fwy[2 * J + 2]);
                     const double dc = 0.5 * (c.wy[J] + c.wy[J + 1]);
-                    c.gN[cid] = (float)(s * df / dc);
+                    c.gN[cid] = (float)(sum * df / dc);
                 }
             }
         }
-    }
+    });
     return c;
 }
 
 std::vector<HostLevel> build_hierarchy(int nx, int ny, int nz, double dx, double dy, const float* gE,
                                        const float* gN, const float* gU, const float* gB, const float* cap) {
+    // Level 0 keeps only its size and cell widths: its arrays are the caller's.
     std::vector<HostLevel> L(1);
-    const size_t n = (size_t)nx * ny * nz, p = (size_t)nx * ny;
     HostLevel& f = L[0];
     f.nx = nx;
     f.ny = ny;
     f.nz = nz;
     f.wx.assign(nx, dx);
     f.wy.assign(ny, dy);
-    f.gE.assign(gE, gE + n);
-    f.gN.assign(gN, gN + n);
-    f.gU.assign(gU, gU + n);
-    f.gB.assign(gB, gB + p);
-    f.cap.assign(cap, cap + n);
-    while (L.back().nx > 1 || L.back().ny > 1) L.push_back(coarsen(L.back()));
+    LevelView v{nx, ny, nz, &L[0].wx, &L[0].wy, gE, gN, gU, gB, cap};
+    while (L.back().nx > 1 || L.back().ny > 1) {
+        HostLevel c = coarsen(v);
+        L.push_back(std::move(c));
+        const HostLevel& b = L.back();
+        v = LevelView{b.nx, b.ny, b.nz, &b.wx, &b.wy, b.gE.data(), b.gN.data(), b.gU.data(), b.gB.data(), b.cap.data()};
+    }
     return L;
 }
 
@@ -108,26 +138,6 @@ ColumnTypes column_types(int nx, int ny, int nz, const float* gE, const float* g
                          const float* gB, const float* cap, int max_types) {
     ColumnTypes out;
     const long cols = (long)nx * ny, plane = cols;
-    // field f of every column on plane l (west / south faces are 0 where the
-    // neighbour is absent), into buf
-    std::vector<float> buf(cols);
-    auto plane_values = [&](int f, int l) -> const float* {
-        const long o = (long)l * plane;
-        if (f == TF_W) {
-            for (int j = 0; j < ny; ++j) {
-                const long r = (long)j * nx;
-                buf[r] = 0.f;
-                std::memcpy(&buf[r + 1], gE + o + r, (size_t)(nx - 1) * sizeof(float));
-            }
-            return buf.data();
-        }
-        if (f == TF_S) {
-            std::fill(buf.begin(), buf.begin() + nx, 0.f);
-            std::memcpy(&buf[nx], gN + o, (size_t)(cols - nx) * sizeof(float));
-            return buf.data();
-        }
-        return (f == TF_E ? gE : f == TF_N ? gN : f == TF_U ? gU : cap) + o;
-    };
     auto value = [&](long c, int f, int l) -> float {  // one value (representatives only)
         const int i = (int)(c % nx), j = (int)(c / nx);
         const long id = (long)l * plane + c;
@@ -145,14 +155,35 @@ ColumnTypes column_types(int nx, int ny, int nz, const float* gE, const float* g
         const int i = (int)(c % nx), j = (int)(c / nx);
         return (unsigned char)((i > 0) | ((i < nx - 1) << 1) | ((j > 0) << 2) | ((j < ny - 1) << 3));
     };
+    // Hashes: rows of columns in parallel, each row's columns plane by plane
+    // and field by field (the same sequence of values per column as a
+    // sequential pass, so the same hashes).
     std::vector<uint64_t> h1(cols);
-    for (long c = 0; c < cols; ++c) h1[c] = mix64(0x243F6A8885A308D3ull, flags_of(c) | ((uint64_t)fbits(gB[c]) << 8));
-    for (int l = 0; l < nz; ++l)  // plane by plane: every array is read in order
-        for (int f = TF_E; f <= TF_S; ++f) {
-            const float* vals = plane_values(f, l);
-            const uint64_t tag = (uint64_t)(f * 64 + l) << 32;
-            for (long c = 0; c < cols; ++c) h1[c] = mix64(h1[c], fbits(vals[c]) | tag);
+    // The six fields of row j's columns on plane l, into v[f][i].
+    auto row_fields = [&](long j, int l, std::vector<float> (&v)[TF_S + 1]) {
+        const long o = (long)l * plane + j * nx;
+        for (int f = TF_E; f <= TF_S; ++f) v[f].resize(nx);
+        std::memcpy(v[TF_E].data(), gE + o, nx * sizeof(float));
+        std::memcpy(v[TF_N].data(), gN + o, nx * sizeof(float));
+        std::memcpy(v[TF_U].data(), gU + o, nx * sizeof(float));
+        std::memcpy(v[TF_C].data(), cap + o, nx * sizeof(float));
+        v[TF_W][0] = 0.f;
+        std::memcpy(v[TF_W].data() + 1, gE + o, (nx - 1) * sizeof(float));
+        if (j > 0) std::memcpy(v[TF_S].data(), gN + o - nx, nx * sizeof(float));
+        else std::fill(v[TF_S].begin(), v[TF_S].end(), 0.f);
+    };
+    parallel_for(ny, [&](long j) {
+        const long r0 = j * nx;
+        std::vector<float> v[TF_S + 1];
+        for (long c = r0; c < r0 + nx; ++c) h1[c] = mix64(0x243F6A8885A308D3ull, flags_of(c) | ((uint64_t)fbits(gB[c]) << 8));
+        for (int l = 0; l < nz; ++l) {
+            row_fields(j, l, v);
+            for (int f = TF_E; f <= TF_S; ++f) {
+                const uint64_t tag = (uint64_t)(f * 64 + l) << 32;
+                for (int i = 0; i < nx; ++i) h1[r0 + i] = mix64(h1[r0 + i], fbits(v[f][i]) | tag);
+            }
         }
+    });
     std::unordered_map<uint64_t, int> ids;
     std::vector<long> rep;
     out.type.resize(cols);
@@ -184,13 +215,23 @@ ColumnTypes column_types(int nx, int ny, int nz, const float* gE, const float* g
         if (out.flags[t] != flags_of(c) || fbits(out.tab[(size_t)t * nz * kTypeFields + TF_B]) != fbits(gB[c]))
             return ColumnTypes{};
     }
-    for (int l = 0; l < nz; ++l)
-        for (int f = TF_E; f <= TF_S; ++f) {
-            const float* vals = plane_values(f, l);
-            const float* col = &out.tab[(size_t)l * kTypeFields + f];  // type t's value: col[t * nz * kTypeFields]
-            for (long c = 0; c < cols; ++c)
-                if (fbits(col[(size_t)out.type[c] * kTypeFields * nz]) != fbits(vals[c])) return ColumnTypes{};
+    std::vector<char> bad(ny, 0);  // per row: a column that differs from its type
+    parallel_for(ny, [&](long j) {
+        std::vector<float> v[TF_S + 1];
+        for (int l = 0; l < nz && !bad[j]; ++l) {
+            row_fields(j, l, v);
+            for (int i = 0; i < nx && !bad[j]; ++i) {
+                const float* tv = &out.tab[((size_t)out.type[j * nx + i] * nz + l) * kTypeFields];
+                for (int f = TF_E; f <= TF_S; ++f)
+                    if (fbits(tv[f]) != fbits(v[f][i])) {
+                        bad[j] = 1;
+                        break;
+                    }
+            }
         }
+    });
+    for (char b : bad)
+        if (b) return ColumnTypes{};
     out.ntypes = nt;
     return out;
 }

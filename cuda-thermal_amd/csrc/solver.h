@@ -109,6 +109,7 @@ struct SolverBase {
     virtual int colorize_into(int l0, int l1, float rise_hi, unsigned* dst, unsigned long long* part,
                               unsigned long long* key, unsigned char* host_rgba, float* hot_rise, int* hot_cell) = 0;
     virtual int device() const = 0;
+    virtual int setup_times(th_setup_times* out) const = 0;
 };
 
 template <class T>
@@ -151,24 +152,46 @@ struct Solver final : SolverBase {
 
     bool mg() const { return opt.precond == TH_PRECOND_MG; }
 
-    // natural-layout host array -> split-layout device array of type D
+    // natural-layout host array -> split-layout device array of type D,
+    // through a staging buffer kept for later uploads (set_x, set_power)
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
     template <class S, class D>
     int upload_split(const Geo& g, int planes, const S* host, D* dev) {
         const long n = (long)g.nx * g.ny * planes;
-        void* tmp = nullptr;
-        HIPTRY(hipMalloc(&tmp, n * sizeof(S)));
-        hipError_t e = hipMemcpyAsync(tmp, host, n * sizeof(S), hipMemcpyHostToDevice, s);
+        if ((size_t)n * sizeof(S) > stage_bytes) {
+            if (stage) {
+                HIPTRY(hipStreamSynchronize(s));
+                owned.erase(std::find(owned.begin(), owned.end(), stage));
+                HIPTRY(hipFree(stage));
+                stage = nullptr;
+            }
+            HIPTRY(hipMalloc(&stage, n * sizeof(S)));
+            owned.push_back(stage);
+            stage_bytes = n * sizeof(S);
+        }
+        hipError_t e = hipMemcpyAsync(stage, host, n * sizeof(S), hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
-            k_to_split<S, D><<<lin_blocks(n), 256, 0, s>>>(g, planes, (const S*)tmp, dev);
+            k_to_split<S, D><<<lin_blocks(n), 256, 0, s>>>(g, planes, (const S*)stage, dev);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(tmp);
         if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
         return TH_OK;
     }
 
+    // Where the set-up time went (GpuSetupTimes, pcg3d.h:101-109), in ms.
+    th_setup_times setup{};
+
     int init(const th_model_desc& m, const th_solver_opts& o) override {
+        using clk = std::chrono::steady_clock;
+        const auto t_start = clk::now();
+        auto t_mark = t_start;
+        auto lap = [&](double& acc) {
+            const auto t = clk::now();
+            acc += std::chrono::duration<double, std::milli>(t - t_mark).count();
+            t_mark = t;
+        };
         opt = o;
         HIPTRY(hipSetDevice(o.device));
         HIPTRY(hipStreamCreate(&s));
@@ -189,6 +212,7 @@ struct Solver final : SolverBase {
         if ((rc = dalloc(&st, 1))) return rc;
         HIPTRY(hipHostMalloc((void**)&h_st, sizeof(State), hipHostMallocDefault));
         std::memset(h_st, 0, sizeof(State));
+        lap(setup.alloc_ms);
 
         // Levels: coarse operators aggregated on the host (levels.cpp), each
         // level stored in the split layout in precision T.
@@ -202,6 +226,7 @@ struct Solver final : SolverBase {
             f.nz = m.nz;
             H.push_back(f);
         }
+        lap(setup.levels_ms);
         L.resize(H.size());
         long maxblk = kMaxVecBlocks;
         for (size_t k = 0; k < H.size(); ++k) {
@@ -211,6 +236,7 @@ struct Solver final : SolverBase {
             for (T** v : {&d.gE, &d.gN, &d.gU, &d.cap, &d.finv, &d.x, &d.b})
                 if ((rc = dalloc(v, nc))) return rc;
             if ((rc = dalloc(&d.gB, d.g.plane))) return rc;
+            lap(setup.alloc_ms);
             const float* src[5];
             if (k == 0) {
                 src[0] = m.gE, src[1] = m.gN, src[2] = m.gU, src[3] = m.cap, src[4] = m.gB;
@@ -221,6 +247,7 @@ struct Solver final : SolverBase {
             T* dst[5] = {d.gE, d.gN, d.gU, d.cap, d.gB};
             for (int a = 0; a < 5; ++a)
                 if ((rc = upload_split<float, T>(d.g, a == 4 ? 1 : d.g.nz, src[a], dst[a]))) return rc;
+            lap(k == 0 ? setup.upload_ms : setup.levels_ms);
             // column types where the level runs the column walk
             if (allow_types && mg() && (long)d.g.hx * d.g.ny > kPPCols) {
                 const ColumnTypes ct = th::column_types(d.g.nx, d.g.ny, d.g.nz, src[0], src[1], src[2], src[4],
@@ -241,6 +268,7 @@ struct Solver final : SolverBase {
                     d.ntypes = ct.ntypes;
                 }
             }
+            lap(setup.types_ms);
             const dim3 cg = col_grid(2 * d.g.hx, d.g.ny);
             maxblk = std::max(maxblk, (long)cg.x * cg.y);
             if (!mg()) break;
@@ -283,13 +311,22 @@ struct Solver final : SolverBase {
                    L.size() - (tail_k0 - 1) <= (size_t)kMaxTail)
                 --tail_k0;
         }
+        lap(setup.alloc_ms);
         if ((rc = upload_split<double, double>(g0, g0.nz, m.P, P))) return rc;
+        lap(setup.upload_ms);
         if ((rc = set_dt(0.0))) return rc;
         HIPTRY(hipStreamSynchronize(s));
+        lap(setup.factor_ms);
         if (opt.use_graphs && mg()) {
             if ((rc = capture(&g_start, true))) return rc;
             if ((rc = capture(&g_iter, false))) return rc;
         }
+        lap(setup.graphs_ms);
+        setup.total_ms = std::chrono::duration<double, std::milli>(clk::now() - t_start).count();
+        return TH_OK;
+    }
+    int setup_times(th_setup_times* out) const override {
+        *out = setup;
         return TH_OK;
     }
 

@@ -151,6 +151,11 @@ int th_solver_heat_out(th_solver* s, double* out) {
 }
 int th_solver_nlevels(th_solver* s) { return s ? s->impl->nlevels() : -1; }
 int th_solver_column_types(th_solver* s) { return s ? s->impl->column_types() : -1; }
+int th_solver_setup_times(th_solver* s, th_setup_times* out) {
+    TH_S(s);
+    if (!out) return fail(TH_ERR_ARG, "null argument");
+    return s->impl->setup_times(out);
+}
 int th_solver_level_info(th_solver* s, int k, int* nx, int* ny, int* nz, int* ntypes, int* kind) {
     TH_S(s);
     return s->impl->level_info(k, nx, ny, nz, ntypes, kind);

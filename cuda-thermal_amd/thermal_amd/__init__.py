@@ -62,6 +62,11 @@ class SolveResult(C.Structure):
                 ("solve_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class SetupTimes(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("alloc_ms", "upload_ms", "levels_ms", "types_ms", "factor_ms", "graphs_ms",
+                                          "total_ms")]
+
+
 class TransientOpts(C.Structure):
     _fields_ = [("dt0", C.c_double), ("growth", C.c_double), ("dt_max", C.c_double), ("t_end", C.c_double),
                 ("tol", C.c_double), ("maxit", C.c_int), ("max_steps", C.c_int), ("predictor", C.c_int)]
@@ -128,6 +133,7 @@ SIGNATURES = {
     "th_solver_nlevels": (C.c_int, [_vp]),
     "th_solver_column_types": (C.c_int, [_vp]),
     "th_solver_level_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip, _ip]),
+    "th_solver_setup_times": (C.c_int, [_vp, C.POINTER(SetupTimes)]),
     "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
@@ -502,6 +508,12 @@ class Solver:
     def column_types(self):
         """Distinct fine-level column types in use (0: per-cell arrays)."""
         return lib().th_solver_column_types(self.h)
+
+    def setup_times(self):
+        """Where the solver's set-up time went, ms (GpuSetupTimes)."""
+        t = SetupTimes()
+        _check(lib().th_solver_setup_times(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in SetupTimes._fields_}
 
     def levels(self):
         """Per multigrid level: dict(nx, ny, nz, types, kind), kind one of

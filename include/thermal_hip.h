@@ -171,6 +171,16 @@ typedef struct {
 } th_stats;          /* PcgStats, pcg3d.h:45-49 */
 
 TH_API int th_solver_create(const th_model_desc* m, const th_solver_opts* o, th_solver** out);
+/* Where th_solver_create's time went, in ms (GpuSetupTimes and
+ * gpu_last_setup, pcg3d.h:101-109; here per solver, not process-global):
+ * device allocations, the fine operator and power map to the device, the
+ * multigrid levels (coarse operators built on the host in parallel, then
+ * re-laid out on the device), the column types (found on the host, their
+ * table to the device), the factors for dt = 0, and the recorded graphs. */
+typedef struct {
+    double alloc_ms, upload_ms, levels_ms, types_ms, factor_ms, graphs_ms, total_ms;
+} th_setup_times;
+TH_API int th_solver_setup_times(th_solver* s, th_setup_times* out);
 TH_API void th_solver_destroy(th_solver* s); /* never fails loudly (pcg3d_gpu.cu:495-509) */
 TH_API int th_solver_set_dt(th_solver* s, double dt);       /* 0 = steady */
 TH_API int th_solver_set_x_zero(th_solver* s);
