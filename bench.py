@@ -189,28 +189,48 @@ def transient_leg():
     return out
 
 
-def transient_config2_leg(model):
+def transient_config2_leg(model, spec):
     """The 1024^2 x 16 TSV stack's 100-step backward-Euler run from ambient
     (dt0 = 10 us growing 10% per step, no cap, tol 1e-9), FP64 and mixed,
-    and mixed with the extrapolated step start: wall time, iterations, worst
-    per-step energy balance, final T against the FP64 run."""
+    and mixed with each step started from the extrapolation of the last two
+    (predictor), from T_old moved towards the surrogate's predicted field
+    (learned: the residual-minimising weight, T_old where none lowers the
+    residual), and from both: wall time, iterations, worst per-step energy
+    balance, final T against the FP64 run."""
+    from thermal_amd import surrogate as TS
+
     P = model.total_power()
     kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, max_steps=100, tol=1e-9)
+    modes = [("fp64", "plain"), ("mixed", "plain"), ("mixed+predictor", "extrapolate")]
+    sur = None
+    if os.path.exists(TS.WEIGHTS):
+        sur = TS.load()
+        dfeats = T.DeviceBuffer(TS.features(spec, TS.GRID))  # kept alive while the prediction is used
+        d_pred = sur.predict_device_input(dfeats.ptr)
+        modes += [("mixed+learned", "learned"), ("mixed+predictor+learned", "extrapolate+learned")]
     out, fields = {}, {}
-    for mode in ("fp64", "mixed", "mixed+predictor"):
+    for mode, pred in modes:
         s = T.Solver(model, precision=mode.split("+")[0])
-        pred = mode.endswith("predictor")
+        if pred.endswith("learned"):
+            s.set_guide_lifted(d_pred, TS.GRID, TS.GRID, 16, TS.COARSE_Z, TS.plane_centres(W.config2_layers()),
+                               scale=T.Surrogate.T_SCALE, passes=1)
         T.run_transient(s, predictor=pred, **dict(kw, max_steps=3))  # warm-up
         g = T.run_transient(s, predictor=pred, **kw)
         out[mode] = dict(ms=round(g["solve_ms"], 2), steps=len(g["t"]), t_end_s=float(g["t"][-1]),
                          iterations=int(g["iters"].sum()), converged=g["converged"],
                          energy_balance_max=float((np.abs(g["q_out"] + g["q_stored"] - P) / P).max()),
                          T_max_C=float(g["T_max"][-1]) + 25.0)
+        if pred.endswith("learned"):
+            used, w = s.guide_stats()
+            out[mode].update(learned_steps=used, learned_w_mean=w)
         fields[mode] = g["T"]
         s.close()
         log(f"  transient 1024^2x16 {mode}: {out[mode]}")
-    out["max_abs_err_mixed_vs_fp64_K"] = float(np.abs(fields["mixed"] - fields["fp64"]).max())
-    out["max_abs_err_predictor_vs_fp64_K"] = float(np.abs(fields["mixed+predictor"] - fields["fp64"]).max())
+    if sur:
+        sur.close()
+    for mode in fields:
+        if mode != "fp64":
+            out[f"max_abs_err_{mode}_vs_fp64_K"] = float(np.abs(fields[mode] - fields["fp64"]).max())
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
     out["workload"] = "1024x1024x16 TSV stack, 100 BE steps from ambient, dt0 10 us x 1.1^k"
     return out
@@ -391,7 +411,7 @@ def main():
     smx.close()
 
     tr = None if args.no_transient else transient_leg()
-    tr2 = None if args.no_transient else transient_config2_leg(model)
+    tr2 = None if args.no_transient else transient_config2_leg(model, W.config2_spec(args.seed))
     learned = None if args.no_learned else learned_leg(model, W.config2_spec(args.seed), args.steps, args.warmup, x64)
     batched = None if args.no_batched else batched_leg()
     side = None

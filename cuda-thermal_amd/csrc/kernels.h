@@ -92,6 +92,10 @@ struct State {
     int refactor;  // 1: the level factors must be rebuilt for inv_dt
     int steps;
     StepRec last;
+    // Learned step start: this step's weight w on the guide direction, and
+    // over the run the steps that took it (w > 0) and the sum of their w.
+    double guide_w, guide_w_sum;
+    int guide_steps;
 };
 
 // Block coordinates of a kernel body: the hardware ones (hw_blk), or, in a

@@ -613,6 +613,9 @@ static __global__ void k_step_init(State* st, double dt0, double growth, double 
     st->t_end = t_end;
     st->heat_prev = 0.0;
     st->steps = 0;
+    st->guide_w = 0.0;
+    st->guide_w_sum = 0.0;
+    st->guide_steps = 0;
 }
 // dt = min(dt, t_end - t) (transient3d.h:60); a changed dt sets the new
 // operator's 1/dt and asks for new factors (set_dt only if dt changed, :61).
@@ -665,6 +668,58 @@ static __global__ __launch_bounds__(kVecThreads) void k_predict(long n, double* 
         x[k] = xo + r * (xo - xprev[k]);
         xprev[k] = xo;
     }
+}
+
+// Learned start of a backward-Euler step, after its right-hand side b = P +
+// C/dt T_old (x = T_old, or the extrapolated start): along the direction
+// d = g - x towards the guide field g (the surrogate's predicted field), the
+// start x + w d with the smallest residual ||b - M (x + w d)||, M = C/dt + A:
+// w = (r0 . M d) / (M d . M d), r0 = b - M x; w <= 0 (no descent along d)
+// keeps x. Per cell, in flux form: M d = M g - M x. Partials: r0.Md, Md.Md,
+// r0.r0 per block.
+template <class T>
+__global__ __launch_bounds__(1024) void k_guide_terms(Geo g, Ops<T> o, const double* __restrict__ b,
+                                                      const double* __restrict__ x, const double* __restrict__ gd,
+                                                      const State* __restrict__ st, double* __restrict__ part) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double v[3] = {0.0, 0.0, 0.0};
+    const double inv_dt = st->inv_dt;
+    int i;
+    if (j < g.ny && q < 2 * g.hx && slot_column(g, q, j, i)) {
+        const Col c = make_col(g, i, j);
+        for (int l = 0; l < g.nz; ++l) {
+            const long id = (long)l * g.plane + c.c0;
+            const double mx = flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, x, c, l, inv_dt);
+            const double mg = flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, gd, c, l, inv_dt);
+            const double r0 = b[id] - mx, md = mg - mx;
+            v[0] += r0 * md;
+            v[1] += md * md;
+            v[2] += r0 * r0;
+        }
+    }
+    block_partials<3>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+}
+// w from k_guide_terms' partials (one block), and the run's tally.
+static __global__ __launch_bounds__(kFinThreads) void k_guide_weight(const double* __restrict__ part, int nblk, State* st) {
+    double t[3];
+    sum_partials<3>(part, nblk, t);
+    if (threadIdx.x != 0) return;
+    const double w = (t[0] > 0.0 && t[1] > 0.0) ? t[0] / t[1] : 0.0;
+    st->guide_w = w;
+    if (w > 0.0) {
+        st->guide_steps += 1;
+        st->guide_w_sum += w;
+    }
+}
+// x += w (g - x).
+static __global__ __launch_bounds__(kVecThreads) void k_guide_step(long n, double* __restrict__ x,
+                                                            const double* __restrict__ gd,
+                                                            const State* __restrict__ st) {
+    const double w = st->guide_w;
+    if (w == 0.0) return;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] += w * (gd[k] - x[k]);
 }
 
 // x = 0 when b = 0 was detected (the reference's set_x_zero on bb == 0).

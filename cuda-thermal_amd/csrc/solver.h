@@ -110,6 +110,10 @@ struct SolverBase {
                               unsigned long long* key, unsigned char* host_rgba, float* hot_rise, int* hot_cell) = 0;
     virtual int device() const = 0;
     virtual int setup_times(th_setup_times* out) const = 0;
+    virtual int set_guide(const double* host) = 0;
+    virtual int set_guide_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale,
+                                 int passes) = 0;
+    virtual int guide_stats(int* steps, double* w_mean) = 0;
 };
 
 template <class T>
@@ -861,17 +865,42 @@ struct Solver final : SolverBase {
     // The host keeps its own copy of t and dt (the same double arithmetic
     // as the device) to know when to stop, and waits once per step.
     double* xprev = nullptr;  // T_older for the step predictor (allocated on first use)
+    double* guide = nullptr;  // guide field of the learned step start (set_guide*)
+    // predictor: 0 T_old (the reference's warm start), 1 extrapolated, 2
+    // learned (T_old moved towards the guide field by the residual-minimising
+    // w >= 0), 3 extrapolated, then learned.
     int enqueue_step_head(double tol, int maxit, int predictor) {
         k_step_begin<<<1, 1, 0, s>>>(st, tol, maxit);
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = enqueue_factors())) return rc;
         if ((rc = set_rhs_transient())) return rc;
-        if (predictor) {
+        if (predictor == 1 || predictor == 3) {
             k_predict<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, xprev, st);
             HIPTRY(hipGetLastError());
         }
+        if (predictor >= 2) {
+            const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
+            k_guide_terms<T><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, guide, st, part);
+            HIPTRY(hipGetLastError());
+            k_guide_weight<<<1, kFinThreads, 0, s>>>(part, (int)(grid.x * grid.y), st);
+            HIPTRY(hipGetLastError());
+            k_guide_step<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, guide, st);
+            HIPTRY(hipGetLastError());
+        }
         return enqueue_start();
+    }
+    int set_guide(const double* host) override {
+        int rc;
+        if (!guide && (rc = dalloc(&guide, n0))) return rc;
+        return upload_split<double, double>(g0, g0.nz, host, guide);
+    }
+    int guide_stats(int* steps, double* w_mean) override {
+        int rc;
+        if ((rc = fetch_state())) return rc;
+        if (steps) *steps = h_st->guide_steps;
+        if (w_mean) *w_mean = h_st->guide_steps ? h_st->guide_w_sum / h_st->guide_steps : 0.0;
+        return TH_OK;
     }
     int enqueue_step_tail() {
         const unsigned vb = vec_blocks(n0);
@@ -895,7 +924,8 @@ struct Solver final : SolverBase {
             if (gh) (void)hipGraphExecDestroy(gh);
             if (gt) (void)hipGraphExecDestroy(gt);
         };
-        if (o.predictor && !xprev && (rc = dalloc(&xprev, n0))) return rc;
+        if ((o.predictor == 1 || o.predictor == 3) && !xprev && (rc = dalloc(&xprev, n0))) return rc;
+        if (o.predictor >= 2 && !guide) return fail(TH_ERR_ARG, "learned step start: no guide field set");
         if (opt.use_graphs && mg()) {
             if ((rc = capture_fn(&gh, [&]() { return enqueue_step_head(o.tol, o.maxit, o.predictor); })) ||
                 (rc = capture_fn(&gt, [&]() { return enqueue_step_tail(); }))) {
@@ -957,6 +987,32 @@ struct Solver final : SolverBase {
         // The lifted guess is built and judged in Ap (free between solves):
         // x is only overwritten when the guess is kept.
         double* v = Ap;
+        if ((rc = lift_into(v, field, Hc, Wc, C, zm, scale, passes))) return rc;
+        if ((rc = relres_of(v, &r_lift))) return rc;
+        const bool keep = r_lift < r_plain;
+        if (keep) {
+            HIPTRY(hipMemcpyAsync(x, v, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+            x_zero = false;
+        }
+        HIPTRY(hipStreamSynchronize(s));
+        if (used) *used = keep;
+        if (rp) *rp = r_plain;
+        if (rl) *rl = r_lift;
+        return TH_OK;
+    }
+    // The guide field of the learned step start from a coarse field, lifted
+    // as set_x_lifted lifts its guess (no judging: each step judges its w).
+    int set_guide_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale,
+                         int passes) override {
+        int rc;
+        if (!guide && (rc = dalloc(&guide, n0))) return rc;
+        if ((rc = lift_into(guide, field, Hc, Wc, C, zm, scale, passes))) return rc;
+        HIPTRY(hipStreamSynchronize(s));
+        return TH_OK;
+    }
+    // v = the coarse field lifted to the fine grid, then `passes` red-black
+    // column line relaxations of the steady operator.
+    int lift_into(double* v, const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes) {
         k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, v);
         HIPTRY(hipGetLastError());
         const Ops<T> o = L[0].ops();
@@ -981,16 +1037,6 @@ struct Solver final : SolverBase {
 #undef TH_RELAX
                 HIPTRY(hipGetLastError());
             }
-        if ((rc = relres_of(v, &r_lift))) return rc;
-        const bool keep = r_lift < r_plain;
-        if (keep) {
-            HIPTRY(hipMemcpyAsync(x, v, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
-            x_zero = false;
-        }
-        HIPTRY(hipStreamSynchronize(s));
-        if (used) *used = keep;
-        if (rp) *rp = r_plain;
-        if (rl) *rl = r_lift;
         return TH_OK;
     }
     int nz() const override { return g0.nz; }

@@ -255,19 +255,15 @@ void th_transient_default_opts(th_transient_opts* o) {
     o->predictor = 0;
 }
 
-int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc, const double* zc_coarse,
-                           const double* zc_fine, double scale, int passes, int* used, double* relres_plain,
-                           double* relres_lift) {
-    TH_S(s);
-    if (!field || !zc_coarse || !zc_fine) return fail(TH_ERR_ARG, "null argument");
-    if (Hc < 1 || Wc < 1 || nzc < 1 || nzc > C || passes < 0) return fail(TH_ERR_ARG, "bad coarse field shape");
-    const int nz = s->impl->nz();
+// Plane-to-channel map of a lift: plane l at zc_fine[l] between the coarse
+// planes at zc_coarse[lo] and zc_coarse[lo + 1] (nearest outside them).
+static LiftMap lift_map(int nz, int nzc, const double* zc_coarse, const double* zc_fine) {
     LiftMap zm{};
     for (int l = 0; l < nz; ++l) {
         const double z = zc_fine[l];
         int k = 0;
         while (k + 1 < nzc && zc_coarse[k + 1] <= z) ++k;
-        if (k + 1 >= nzc || z <= zc_coarse[0]) {  // outside the coarse centres: nearest plane
+        if (k + 1 >= nzc || z <= zc_coarse[0]) {
             zm.lo[l] = z <= zc_coarse[0] ? 0 : nzc - 1;
             zm.w[l] = 0.0f;
         } else {
@@ -275,6 +271,35 @@ int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int
             zm.w[l] = (float)((z - zc_coarse[k]) / (zc_coarse[k + 1] - zc_coarse[k]));
         }
     }
+    return zm;
+}
+
+int th_solver_set_guide(th_solver* s, const double* G) {
+    TH_S(s);
+    if (!G) return fail(TH_ERR_ARG, "null argument");
+    return s->impl->set_guide(G);
+}
+int th_solver_set_guide_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc,
+                               const double* zc_coarse, const double* zc_fine, double scale, int passes) {
+    TH_S(s);
+    if (!field || !zc_coarse || !zc_fine) return fail(TH_ERR_ARG, "null argument");
+    if (Hc < 1 || Wc < 1 || nzc < 1 || nzc > C || passes < 0) return fail(TH_ERR_ARG, "bad coarse field shape");
+    return s->impl->set_guide_lifted(field, Hc, Wc, C, lift_map(s->impl->nz(), nzc, zc_coarse, zc_fine), scale,
+                                     passes);
+}
+int th_solver_guide_stats(th_solver* s, int* steps, double* w_mean) {
+    TH_S(s);
+    return s->impl->guide_stats(steps, w_mean);
+}
+
+int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc, const double* zc_coarse,
+                           const double* zc_fine, double scale, int passes, int* used, double* relres_plain,
+                           double* relres_lift) {
+    TH_S(s);
+    if (!field || !zc_coarse || !zc_fine) return fail(TH_ERR_ARG, "null argument");
+    if (Hc < 1 || Wc < 1 || nzc < 1 || nzc > C || passes < 0) return fail(TH_ERR_ARG, "bad coarse field shape");
+    const int nz = s->impl->nz();
+    const LiftMap zm = lift_map(nz, nzc, zc_coarse, zc_fine);
     return s->impl->set_x_lifted(field, Hc, Wc, C, zm, scale, passes, used, relres_plain, relres_lift);
 }
 
@@ -283,7 +308,7 @@ int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* r
                      double* T_final, int* all_converged, double* solve_ms) {
     TH_S(s);
     if (!o || !nsteps) return fail(TH_ERR_ARG, "null argument");
-    if (!(o->dt0 > 0) || !(o->growth > 0) || !(o->dt_max > 0) || o->maxit < 0 || o->predictor < 0 || o->predictor > 1)
+    if (!(o->dt0 > 0) || !(o->growth > 0) || !(o->dt_max > 0) || o->maxit < 0 || o->predictor < 0 || o->predictor > 3)
         return fail(TH_ERR_ARG, "transient options out of range");
     int rc = s->impl->run_transient(*o, recs, rec_cap, nsteps, all_converged, solve_ms);
     if (rc) return rc;

@@ -134,6 +134,10 @@ SIGNATURES = {
     "th_solver_column_types": (C.c_int, [_vp]),
     "th_solver_level_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip, _ip]),
     "th_solver_setup_times": (C.c_int, [_vp, C.POINTER(SetupTimes)]),
+    "th_solver_set_guide": (C.c_int, [_vp, _dp]),
+    "th_solver_set_guide_lifted": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                             C.c_int]),
+    "th_solver_guide_stats": (C.c_int, [_vp, _ip, _dp]),
     "th_solver_colorize": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
     "th_solver_colorize_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, _vp, _fp, _ip]),
@@ -515,6 +519,25 @@ class Solver:
         _check(lib().th_solver_setup_times(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in SetupTimes._fields_}
 
+    def set_guide(self, G):
+        """Guide field of the learned step start (natural layout, K)."""
+        self._guide = np.ascontiguousarray(G, np.float64)
+        _check(lib().th_solver_set_guide(self.h, _d(self._guide)))
+
+    def set_guide_lifted(self, d_field, Hc, Wc, C_, zc_coarse, zc_fine, scale=1.0, passes=2):
+        """Guide field from a coarse device field (the surrogate's output),
+        lifted as set_x_lifted lifts its guess."""
+        zc = np.ascontiguousarray(zc_coarse, np.float64)
+        zf = np.ascontiguousarray(zc_fine, np.float64)
+        _check(lib().th_solver_set_guide_lifted(self.h, d_field, Hc, Wc, C_, len(zc), _d(zc), _d(zf), scale, passes))
+
+    def guide_stats(self):
+        """(steps whose start took the guide direction, their mean w) over
+        the last run_transient."""
+        n, w = C.c_int(), C.c_double()
+        _check(lib().th_solver_guide_stats(self.h, C.byref(n), C.byref(w)))
+        return n.value, w.value
+
     def levels(self):
         """Per multigrid level: dict(nx, ny, nz, types, kind), kind one of
         "column walk", "plane-parallel", "tail"."""
@@ -588,10 +611,18 @@ def steady_solve(model, tol=1e-10, maxit=50000, **kw):
                 solve_ms=r.solve_ms, ms=r.total_ms)
 
 
+PREDICTORS = {"plain": 0, "extrapolate": 1, "learned": 2, "extrapolate+learned": 3}
+
+
 def run_transient(solver, dt0=1e-5, growth=1.1, dt_max=0.05, t_end=1.0, tol=1e-9, maxit=50000, max_steps=0,
                   rec_cap=100000, predictor=False):
     """run_transient (transient3d.h:48-90) on a Solver; energy terms on the device.
-    predictor: start each step from the linear extrapolation of the last two."""
+    predictor: the start of each step - False / "plain" (T_old), True /
+    "extrapolate" (linear extrapolation of the last two steps), "learned"
+    (T_old moved towards the solver's guide field, see Solver.set_guide),
+    "extrapolate+learned"."""
+    if isinstance(predictor, str):
+        predictor = PREDICTORS[predictor]
     o = TransientOpts(dt0, growth, dt_max, t_end, tol, maxit, max_steps, int(predictor))
     recs = (StepRecord * rec_cap)()
     n, conv = C.c_int(), C.c_int()

@@ -274,7 +274,13 @@ typedef struct {
     int maxit;
     int max_steps;                     /* > 0: also stop after this many steps */
     int predictor;                     /* start of each step: 0 = T_old (the reference's warm
-                                          start); 1 = linear extrapolation of the last two steps */
+                                          start); 1 = linear extrapolation of the last two steps;
+                                          2 = learned: T_old + w (G - T_old) towards the guide field
+                                          G (th_solver_set_guide*, e.g. the surrogate's predicted
+                                          field), w >= 0 minimising the step's starting residual
+                                          ||b - (C/dt + A) x||, w = 0 (T_old, the plain warm start)
+                                          when no w > 0 lowers it; 3 = 1, then 2 from the
+                                          extrapolated start */
 } th_transient_opts;
 TH_API void th_transient_default_opts(th_transient_opts* o);
 
@@ -290,6 +296,16 @@ typedef struct {
 TH_API int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* recs,
                             int rec_cap, int* nsteps, double* T_final, int* all_converged,
                             double* solve_ms);
+/* The guide field G of the learned step start (no counterpart in the
+ * reference): from a host array (natural layout, K above ambient), or from a
+ * coarse device field lifted as th_solver_set_x_lifted lifts its guess (for
+ * the surrogate's output; the same arguments, without the judging). */
+TH_API int th_solver_set_guide(th_solver* s, const double* G);
+TH_API int th_solver_set_guide_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc,
+                                      const double* zc_coarse, const double* zc_fine, double scale, int passes);
+/* Over the last th_run_transient: the steps whose start took the guide
+ * direction (w > 0) and their mean w. */
+TH_API int th_solver_guide_stats(th_solver* s, int* steps, double* w_mean);
 
 /* ---- batched steady solves (no reference counterpart) ---------------------
  * Many independent stacks (grid sizes may differ; plane counts <= 16 in one
