@@ -78,66 +78,110 @@ static long total_params() {
 
 enum { EPI_RELU = 0, EPI_LINEAR = 1, EPI_F32 = 2 };
 
-// Y = epi(conv(X, W) + b); Wf[tap][cout][cin].
-template <int KS, int CIN, int COUT, int EPI>
+// Y = epi(conv(X, W) + b); Wf[tap][cout][cin]. Each wavefront computes MT
+// tiles of 16 consecutive pixels for every output channel: per tap and
+// K-slice it loads the MT input fragments and the COUT / 16 weight
+// fragments once and issues MT x COUT / 16 MFMAs, so every weight fragment
+// feeds MT of them (with MT = 1 a weight fragment read through L1 fed one
+// MFMA, and the kernel was bound by those reads, not by the matrix cores).
+// Each output's MFMA chain runs over (tap, K-slice) in the same order for
+// any MT, so the results do not depend on it.
+template <int KS, int CIN, int COUT, int EPI, int MT = 4>
 __global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf* __restrict__ Wf,
                                               const float* __restrict__ bias, void* __restrict__ Yv, long npix,
                                               int H, int W) {
     constexpr int KC = (CIN % 32 == 0) ? 32 : 16;  // K of one MFMA
     constexpr int NT = COUT / 16;
     const int lane = threadIdx.x & 63;
-    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * MT);
     if (p0 >= npix) return;
     const int m = lane & 15, q = lane >> 4;
-    const long pm = p0 + m;
-    const bool valid = pm < npix;
     const long hw = (long)H * W;
-    const long img = valid ? pm / hw : 0;
-    const int rem = valid ? (int)(pm - img * hw) : 0;
-    const int y = rem / W, x = rem - (rem / W) * W;
-    f4v acc[NT];
+    long img[MT];
+    int py[MT], px[MT];
+    bool valid[MT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < MT; ++u) {
+        const long pm = p0 + 16 * u + m;
+        valid[u] = pm < npix;
+        img[u] = valid[u] ? pm / hw : 0;
+        const int rem = valid[u] ? (int)(pm - img[u] * hw) : 0;
+        py[u] = rem / W;
+        px[u] = rem - py[u] * W;
+    }
+    f4v acc[MT][NT];
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[u][t] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int tap = 0; tap < KS * KS; ++tap) {
-        const int yy = y + tap / KS - KS / 2, xx = x + tap % KS - KS / 2;
-        const bool inb = valid && yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const bf* xp = X + (img * hw + (long)yy * W + xx) * CIN;
+        const bf* xp[MT];
+        bool inb[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            const int yy = py[u] + tap / KS - KS / 2, xx = px[u] + tap % KS - KS / 2;
+            inb[u] = valid[u] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            xp[u] = X + (img[u] * hw + (long)(inb[u] ? yy : 0) * W + (inb[u] ? xx : 0)) * CIN;
+        }
         const bf* wp = Wf + ((long)tap * COUT + m) * CIN;
 #pragma unroll
         for (int c0 = 0; c0 < CIN; c0 += KC) {
             if constexpr (KC == 32) {
-                const u4v av = inb ? *(const u4v*)(xp + c0 + 8 * q) : u4v{0u, 0u, 0u, 0u};
-                const b8v a = __builtin_bit_cast(b8v, av);
+                b8v a[MT];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    const u4v av = inb[u] ? *(const u4v*)(xp[u] + c0 + 8 * q) : u4v{0u, 0u, 0u, 0u};
+                    a[u] = __builtin_bit_cast(b8v, av);
+                }
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
-                    const u4v bv = *(const u4v*)(wp + (long)t * 16 * CIN + c0 + 8 * q);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(b8v, bv), acc[t], 0, 0, 0);
+                    const b8v bw = __builtin_bit_cast(b8v, *(const u4v*)(wp + (long)t * 16 * CIN + c0 + 8 * q));
+#pragma unroll
+                    for (int u = 0; u < MT; ++u)
+                        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw, a[u], acc[u][t], 0, 0, 0);
                 }
             } else {
-                const u2v av = inb ? *(const u2v*)(xp + c0 + 4 * q) : u2v{0u, 0u};
-                const s4v a = __builtin_bit_cast(s4v, av);
+                s4v a[MT];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) {
+                    const u2v av = inb[u] ? *(const u2v*)(xp[u] + c0 + 4 * q) : u2v{0u, 0u};
+                    a[u] = __builtin_bit_cast(s4v, av);
+                }
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
-                    const u2v bv = *(const u2v*)(wp + (long)t * 16 * CIN + c0 + 4 * q);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, __builtin_bit_cast(s4v, bv), acc[t], 0, 0, 0);
+                    const s4v bw = __builtin_bit_cast(s4v, *(const u2v*)(wp + (long)t * 16 * CIN + c0 + 4 * q));
+#pragma unroll
+                    for (int u = 0; u < MT; ++u)
+                        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw, a[u], acc[u][t], 0, 0, 0);
                 }
             }
         }
     }
-    // lane (m, q) holds rows (pixels) 4q + i, column (channel) m of each tile
+    // The weights are the MFMA's A operand (rows: output channels) and the
+    // input its B operand (columns: pixels), so lane (m, q) holds output
+    // channels 4q .. 4q + 3 of tile t for pixel m: one 8-byte (BF16) or
+    // 16-byte (FP32) store per tile instead of four 2-byte ones.
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + m;
-        const float bb = (EPI == EPI_LINEAR) ? 0.f : bias[co];
+    for (int u = 0; u < MT; ++u) {
+        const long pr = p0 + 16 * u + m;
+        if (pr >= npix) continue;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const long pr = p0 + 4 * q + i;
-            if (pr >= npix) continue;
-            float v = acc[t][i] + bb;
-            if (EPI == EPI_RELU) v = fmaxf(v, 0.f);
-            if (EPI == EPI_F32) ((float*)Yv)[pr * COUT + co] = v;
-            else ((bf*)Yv)[pr * COUT + co] = f2bf(v);
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + 4 * q;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[u][t][i] + (EPI == EPI_LINEAR ? 0.f : bias[co + i]);
+                if (EPI == EPI_RELU) v[i] = fmaxf(v[i], 0.f);
+            }
+            if (EPI == EPI_F32) {
+                *(f4v*)((float*)Yv + pr * COUT + co) = f4v{v[0], v[1], v[2], v[3]};
+            } else {
+                const u2v w2{(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
+                             (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
+                *(u2v*)((bf*)Yv + pr * COUT + co) = w2;
+            }
         }
     }
 }
@@ -215,13 +259,26 @@ __global__ void k_reduce_grad(const float* __restrict__ part, int nchunks, int K
     else gb[k - COUT * KK * CIN] = s;
 }
 
+// Elementwise kernels over NHWC tensors, 8 channels (16 bytes) per thread:
+// C is a multiple of 8, so a thread's 8 values share one pixel. Each value
+// is computed as the one-value-per-thread form would (the same float
+// operations and rounding).
+typedef unsigned short bf8 __attribute__((ext_vector_type(8)));
+
 __global__ void k_relu_bwd(bf* __restrict__ g, const bf* __restrict__ y, long n) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && !(bf2f(y[k]) > 0.f)) g[k] = 0;
+    const long k = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    if (k >= n) return;
+    bf8 gv = *(const bf8*)(g + k);
+    const bf8 yv = *(const bf8*)(y + k);
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        if (!(bf2f(yv[e]) > 0.f)) gv[e] = 0;
+    *(bf8*)(g + k) = gv;
 }
 
+// Y = average of the 2x2 children of X.
 __global__ void k_pool(const bf* __restrict__ X, bf* __restrict__ Y, long nout, int Ho, int Wo, int C) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8;
     if (k >= nout) return;
     const int c = (int)(k % C);
     const long pix = k / C;
@@ -231,14 +288,18 @@ __global__ void k_pool(const bf* __restrict__ X, bf* __restrict__ Y, long nout, 
     const long img = r / Ho;
     const int W = 2 * Wo;
     const long base = ((img * 2 * Ho + 2 * yo) * (long)W + 2 * xo) * C + c;
-    const float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (long)W * C]) + bf2f(X[base + (long)W * C + C]);
-    Y[k] = f2bf(0.25f * s);
+    const bf8 a = *(const bf8*)(X + base), b = *(const bf8*)(X + base + C);
+    const bf8 cc = *(const bf8*)(X + base + (long)W * C), d = *(const bf8*)(X + base + (long)W * C + C);
+    bf8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = f2bf(0.25f * (bf2f(a[e]) + bf2f(b[e]) + bf2f(cc[e]) + bf2f(d[e])));
+    *(bf8*)(Y + k) = o;
 }
 
 // Y = up(Xc) + S (nearest, x2).
 __global__ void k_upadd(const bf* __restrict__ Xc, const bf* __restrict__ S, bf* __restrict__ Y, long n, int H, int W,
                         int C) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8;
     if (k >= n) return;
     const int c = (int)(k % C);
     const long pix = k / C;
@@ -247,12 +308,16 @@ __global__ void k_upadd(const bf* __restrict__ Xc, const bf* __restrict__ S, bf*
     const int y = (int)(r % H);
     const long img = r / H;
     const long ck = ((img * (H / 2) + y / 2) * (long)(W / 2) + x / 2) * C + c;
-    Y[k] = f2bf(bf2f(Xc[ck]) + bf2f(S[k]));
+    const bf8 a = *(const bf8*)(Xc + ck), b = *(const bf8*)(S + k);
+    bf8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = f2bf(bf2f(a[e]) + bf2f(b[e]));
+    *(bf8*)(Y + k) = o;
 }
 
 // Gc = sum over the 2x2 children of G.
 __global__ void k_upbwd(const bf* __restrict__ G, bf* __restrict__ Gc, long nc, int Hc, int Wc, int C) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8;
     if (k >= nc) return;
     const int c = (int)(k % C);
     const long pix = k / C;
@@ -262,13 +327,18 @@ __global__ void k_upbwd(const bf* __restrict__ G, bf* __restrict__ Gc, long nc, 
     const long img = r / Hc;
     const int W = 2 * Wc;
     const long base = ((img * 2 * Hc + 2 * yo) * (long)W + 2 * xo) * C + c;
-    Gc[k] = f2bf(bf2f(G[base]) + bf2f(G[base + C]) + bf2f(G[base + (long)W * C]) + bf2f(G[base + (long)W * C + C]));
+    const bf8 a = *(const bf8*)(G + base), b = *(const bf8*)(G + base + C);
+    const bf8 cc = *(const bf8*)(G + base + (long)W * C), d = *(const bf8*)(G + base + (long)W * C + C);
+    bf8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = f2bf(bf2f(a[e]) + bf2f(b[e]) + bf2f(cc[e]) + bf2f(d[e]));
+    *(bf8*)(Gc + k) = o;
 }
 
 // G = 0.25 up(Gp) + Gs: gradient of a skip tensor that also fed a pool.
 __global__ void k_poolbwd_add(const bf* __restrict__ Gp, const bf* __restrict__ Gs, bf* __restrict__ G, long n, int H,
                               int W, int C) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 8;
     if (k >= n) return;
     const int c = (int)(k % C);
     const long pix = k / C;
@@ -277,7 +347,11 @@ __global__ void k_poolbwd_add(const bf* __restrict__ Gp, const bf* __restrict__ 
     const int y = (int)(r % H);
     const long img = r / H;
     const long ck = ((img * (H / 2) + y / 2) * (long)(W / 2) + x / 2) * C + c;
-    G[k] = f2bf(0.25f * bf2f(Gp[ck]) + bf2f(Gs[k]));
+    const bf8 a = *(const bf8*)(Gp + ck), b = *(const bf8*)(Gs + k);
+    bf8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = f2bf(0.25f * bf2f(a[e]) + bf2f(b[e]));
+    *(bf8*)(G + k) = o;
 }
 
 // Network input of a batch from the stored features: 16 channels, 3 used.
@@ -536,14 +610,22 @@ struct Net {
     }
 
     // ---- launches ---------------------------------------------------------
+    // 64 pixels per wave where that still gives every SIMD (1024 on the chip)
+    // four waves; 16 otherwise (a batch of one 256 x 256 image at the top
+    // level would give each SIMD a single wave).
+    static bool conv_wide(long npix) { return npix >= 64L * 4 * 1024; }
     int conv(int k, const bf* in, void* outp, int B, int epi) {
         const LayerSpec& l = kSpec[k];
         const long n = npix(l.level, B);
         const int h = H >> l.level, w = W >> l.level;
-        const unsigned g = grid1(n, 64);
+        const bool wide = conv_wide(n);
+        const unsigned g = grid1(n, wide ? 256 : 64);  // 4 waves of 64 or 16 pixels
         const float* bias = prm + off[k] + (long)l.cout * l.ks * l.ks * l.cin;
-#define SUR_CONV(KS, CI, CO, EPI) \
-    k_conv<KS, CI, CO, EPI><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w)
+#define SUR_CONV(KS, CI, CO, EPI)                                                              \
+    do {                                                                                       \
+        if (wide) k_conv<KS, CI, CO, EPI, 4><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w); \
+        else k_conv<KS, CI, CO, EPI, 1><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w);      \
+    } while (0)
         if (epi == EPI_F32) {
             if (l.ks == 1 && l.cin == 32 && l.cout == 16) SUR_CONV(1, 32, 16, EPI_F32);
             else return fail(TH_ERR_ARG, "surrogate: no FP32-output conv for this shape");
@@ -564,9 +646,13 @@ struct Net {
         const LayerSpec& l = kSpec[k];
         const long n = npix(l.level, B);
         const int h = H >> l.level, w = W >> l.level;
-        const unsigned g = grid1(n, 64);
-#define SUR_DG(KS, CI, CO) \
-    k_conv<KS, CI, CO, EPI_LINEAR><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w)
+        const bool wide = conv_wide(n);
+        const unsigned g = grid1(n, wide ? 256 : 64);
+#define SUR_DG(KS, CI, CO)                                                                              \
+    do {                                                                                                \
+        if (wide) k_conv<KS, CI, CO, EPI_LINEAR, 4><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w); \
+        else k_conv<KS, CI, CO, EPI_LINEAR, 1><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w);      \
+    } while (0)
         if (l.ks == 1 && l.cout == 16 && l.cin == 32) SUR_DG(1, 16, 32);
         else if (l.ks == 3 && l.cout == 32 && l.cin == 32) SUR_DG(3, 32, 32);
         else if (l.ks == 3 && l.cout == 64 && l.cin == 32) SUR_DG(3, 64, 32);
@@ -602,31 +688,31 @@ struct Net {
         return TH_OK;
     }
     int relu_bwd(bf* g, const bf* y, long n) {
-        k_relu_bwd<<<grid1(n), 256, 0, s>>>(g, y, n);
+        k_relu_bwd<<<grid1(n / 8), 256, 0, s>>>(g, y, n);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
     int pool(const bf* x, bf* y, int lvl_out, int B, int C) {
         const long n = npix(lvl_out, B) * C;
-        k_pool<<<grid1(n), 256, 0, s>>>(x, y, n, H >> lvl_out, W >> lvl_out, C);
+        k_pool<<<grid1(n / 8), 256, 0, s>>>(x, y, n, H >> lvl_out, W >> lvl_out, C);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
     int upadd(const bf* xc, const bf* sk, bf* y, int lvl, int B, int C) {
         const long n = npix(lvl, B) * C;
-        k_upadd<<<grid1(n), 256, 0, s>>>(xc, sk, y, n, H >> lvl, W >> lvl, C);
+        k_upadd<<<grid1(n / 8), 256, 0, s>>>(xc, sk, y, n, H >> lvl, W >> lvl, C);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
     int upbwd(const bf* g, bf* gc, int lvl_c, int B, int C) {
         const long n = npix(lvl_c, B) * C;
-        k_upbwd<<<grid1(n), 256, 0, s>>>(g, gc, n, H >> lvl_c, W >> lvl_c, C);
+        k_upbwd<<<grid1(n / 8), 256, 0, s>>>(g, gc, n, H >> lvl_c, W >> lvl_c, C);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
     int poolbwd_add(const bf* gp, const bf* gs, bf* g, int lvl, int B, int C) {
         const long n = npix(lvl, B) * C;
-        k_poolbwd_add<<<grid1(n), 256, 0, s>>>(gp, gs, g, n, H >> lvl, W >> lvl, C);
+        k_poolbwd_add<<<grid1(n / 8), 256, 0, s>>>(gp, gs, g, n, H >> lvl, W >> lvl, C);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
