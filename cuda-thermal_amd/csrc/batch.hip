@@ -15,10 +15,10 @@
 // levels in LDS, and climbs back. Start and iteration are captured as graphs.
 //
 // Setup reuses th::Solver<float> per instance (levels, factors, buffers).
-// The autotuner times the V-cycle stages for 1, 2, 4 or 8 rows of 64
-// columns per block (one wavefront per 64 columns of a row) and the vector
-// stages for 256- or 1024-thread blocks on the batch itself, and keeps the
-// fastest.
+// The autotuner times the V-cycle stages for blocks of 1, 2, 4 or 8
+// wavefronts whose wavefront covers 64 columns of one row, 32 of two or 16
+// of four (columns per wavefront), and the vector stages for 256- or
+// 1024-thread blocks, on the batch itself, and keeps the fastest.
 #include "solver.h"
 
 namespace th {
@@ -206,7 +206,7 @@ struct Batch {
     std::vector<void*> own;
     int n = 0, nz = 0, maxk = 0;  // maxk: deepest level with separate kernels + 1
     size_t lds = 0;
-    int rows = 4, vthreads = 1024;
+    int rows = 4, cols = 64, vthreads = 1024;  // V-cycle blocks: rows x (64 / cols) waves of cols columns
     Table t_stencil, t_vec;
     std::vector<Table> t_sweep, t_restrict;
     hipGraphExec_t g_start = nullptr, g_iter = nullptr;
@@ -315,7 +315,7 @@ struct Batch {
         HIPTRY(hipFuncSetAttribute((const void*)kb_tail<8>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
         HIPTRY(hipFuncSetAttribute((const void*)kb_tail<12>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
         HIPTRY(hipFuncSetAttribute((const void*)kb_tail<16>, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes));
-        return configure(4, 1024);
+        return configure(4, 64, 1024);
     }
     std::vector<Inst> hInst;
 
@@ -331,14 +331,17 @@ struct Batch {
         return TH_OK;
     }
 
-    // Tables and graphs for `r` rows of 64 columns per V-cycle block and
-    // `vt`-thread vector blocks.
-    int configure(int r, int vt) {
+    // Tables and graphs for V-cycle blocks of cw columns by r * 64 / cw rows
+    // (r wavefronts, each cw columns of 64 / cw rows) and `vt`-thread vector
+    // blocks.
+    int configure(int r, int cw, int vt) {
         HIPTRY(hipStreamSynchronize(s));
         drop_graphs();
         free_tables();
         rows = r;
+        cols = cw;
         vthreads = vt;
+        const int br = r * kColsX / cw;  // rows of a block
         t_sweep.resize(maxk);
         t_restrict.resize(maxk);
         for (int i = 0; i < n; ++i) {
@@ -351,8 +354,8 @@ struct Batch {
             for (int k = 0; k < d.tail_k0; ++k) {
                 const Geo& g = d.L[k].g;
                 const Geo& c = d.L[k + 1].g;
-                add(t_sweep[k], i, (g.hx + kColsX - 1) / kColsX, (g.ny + r - 1) / r, 1);
-                add(t_restrict[k], i, (c.nx + kColsX - 1) / kColsX, (c.ny + r - 1) / r, c.nz);
+                add(t_sweep[k], i, (g.hx + cw - 1) / cw, (g.ny + br - 1) / br, 1);
+                add(t_restrict[k], i, (c.nx + cw - 1) / cw, (c.ny + br - 1) / br, c.nz);
             }
         }
         HIPTRY(hipMemcpy(dI, hInst.data(), n * sizeof(Inst), hipMemcpyHostToDevice));
@@ -385,7 +388,7 @@ struct Batch {
     int sweep(int k, int colour, int cons) {
         const Table& t = t_sweep[k];
         if (!t.blocks) return TH_OK;
-        const dim3 blk(kColsX, rows);
+        const dim3 blk(cols, rows * kColsX / cols);
 #define TB_SW(NZ) kb_sweep<NZ, MODE><<<t.blocks, blk, 0, s>>>(dI, t.d, (int)t.h.size(), k, colour, cons)
         switch (bucket(nz)) {
             case 4: TB_SW(4); break;
@@ -400,7 +403,7 @@ struct Batch {
     int restrict_(int k) {
         const Table& t = t_restrict[k];
         if (!t.blocks) return TH_OK;
-        kb_restrict<<<t.blocks, dim3(kColsX, rows), 0, s>>>(dI, t.d, (int)t.h.size(), k);
+        kb_restrict<<<t.blocks, dim3(cols, rows * kColsX / cols), 0, s>>>(dI, t.d, (int)t.h.size(), k);
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
@@ -513,10 +516,11 @@ struct Batch {
         return TH_OK;
     }
 
-    // Times the V-cycle (rows) and the vector stages (threads) on the batch
-    // and keeps the fastest shapes. The solution vectors are left arbitrary
-    // (each solve starts from T = 0).
-    int autotune(int reps, int* best_rows, int* best_vt, double* best_ms) {
+    // Times the V-cycle for every block shape (1, 2, 4 or 8 wavefronts of
+    // 64, 32 or 16 columns of a row each) and the vector stages (256- or
+    // 1024-thread blocks) on the batch, and keeps the fastest. The solution
+    // vectors are left arbitrary (each solve starts from T = 0).
+    int autotune(int reps, int* best_rows, int* best_cols, int* best_vt, double* best_ms) {
         hipEvent_t e0, e1;
         HIPTRY(hipEventCreate(&e0));
         HIPTRY(hipEventCreate(&e1));
@@ -535,17 +539,19 @@ struct Batch {
             *out = ms / reps;
             return TH_OK;
         };
-        int rc = TH_OK, br = 4, bv = 1024;
+        int rc = TH_OK, br = 4, bc = 64, bv = 1024;
         double bt = 1e30, vbest = 1e30;
-        for (int r : {1, 2, 4, 8}) {
-            if ((rc = configure(r, 1024))) break;
-            double t = 0;
-            if ((rc = timed([&]() { return vcycle(CONS_NONE); }, &t))) break;
-            if (t < bt) bt = t, br = r;
-        }
+        for (int cw : {64, 32, 16})
+            for (int r : {1, 2, 4, 8}) {
+                if (rc) break;
+                if ((rc = configure(r, cw, 1024))) break;
+                double t = 0;
+                if ((rc = timed([&]() { return vcycle(CONS_NONE); }, &t))) break;
+                if (t < bt) bt = t, br = r, bc = cw;
+            }
         for (int vt : {256, 1024}) {
             if (rc) break;
-            if ((rc = configure(br, vt))) break;
+            if ((rc = configure(br, bc, vt))) break;
             double t = 0;
             if ((rc = timed([&]() {
                      int q;
@@ -558,8 +564,9 @@ struct Batch {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
-        if ((rc = configure(br, bv))) return rc;
+        if ((rc = configure(br, bc, bv))) return rc;
         if (best_rows) *best_rows = br;
+        if (best_cols) *best_cols = bc;
         if (best_vt) *best_vt = bv;
         if (best_ms) *best_ms = bt;
         return TH_OK;
@@ -627,15 +634,16 @@ int th_batch_get_x(th_batch* b, int i, double* x) {
     return b->b.S[i]->get_x(x);
 }
 
-int th_batch_autotune(th_batch* b, int reps, int* rows, int* vec_threads, double* vcycle_ms) {
+int th_batch_autotune(th_batch* b, int reps, int* rows, int* cols, int* vec_threads, double* vcycle_ms) {
     if (!b || reps < 1) return fail(TH_ERR_ARG, "bad argument");
     clear_error();
-    return b->b.autotune(reps, rows, vec_threads, vcycle_ms);
+    return b->b.autotune(reps, rows, cols, vec_threads, vcycle_ms);
 }
 
-int th_batch_shape(th_batch* b, int* rows, int* vec_threads) {
+int th_batch_shape(th_batch* b, int* rows, int* cols, int* vec_threads) {
     if (!b) return fail(TH_ERR_ARG, "null batch");
     if (rows) *rows = b->b.rows;
+    if (cols) *cols = b->b.cols;
     if (vec_threads) *vec_threads = b->b.vthreads;
     return TH_OK;
 }

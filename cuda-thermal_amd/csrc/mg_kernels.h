@@ -257,13 +257,13 @@ __device__ __forceinline__ void restrict_cell_fast(const Geo& f, const T* __rest
     cb[(long)l * cg.plane + cg.pos(I, J)] = restrict_sum_fast<T>(f, gE, gN, fx, I, J, l);
 }
 
-// Block (B.x, B.y) covers kColsX x blockDim.y coarse cells of plane B.z.
+// Block (B.x, B.y) covers blockDim.x x blockDim.y coarse cells of plane B.z.
 template <class T>
 __device__ __forceinline__ void restrict_fast_body(const Blk& B, const Geo& f, const Ops<T>& o,
                                                    const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb,
                                                    const State* __restrict__ st) {
     if (st->done) return;
-    const int I = B.x * kColsX + threadIdx.x;
+    const int I = B.x * blockDim.x + threadIdx.x;
     const int J = B.y * blockDim.y + threadIdx.y;
     if (I >= cg.nx || J >= cg.ny) return;
     restrict_cell_fast<T>(f, o.gE, o.gN, fx, cg, cb, I, J, B.z);
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_restrict_fast(Geo f, Ops<T> o, const T*
     restrict_fast_body<T>(hw_blk(), f, o, fx, cg, cb, st);
 }
 
-// One colour sweep of a level: block (B.x, B.y) covers kColsX columns of the
+// One colour sweep of a level: block (B.x, B.y) covers blockDim.x columns of the
 // colour in each of blockDim.y rows. The first sweep of a V-cycle can also
 // carry the convergence test of the kernel before it (cons_mode, see
 // consume_convergence): a converged solve ends here, with no separate launch.
@@ -285,7 +285,7 @@ __device__ __forceinline__ void sweep_body(const Blk& B, const Geo& g, const Ops
                                            int cons_nblk, int cons_mode) {
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
-    const int t = B.x * kColsX + threadIdx.x;
+    const int t = B.x * blockDim.x + threadIdx.x;
     const int j = B.y * blockDim.y + threadIdx.y;
     if (j >= g.ny) return;
     const int i = 2 * t + ((j + colour) & 1);

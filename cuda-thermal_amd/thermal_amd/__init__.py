@@ -155,8 +155,8 @@ SIGNATURES = {
     "th_batch_set_power": (C.c_int, [_vp, C.c_int, _dp]),
     "th_batch_solve": (C.c_int, [_vp, C.c_double, C.c_int, _ip, _dp, _ip, _dp]),
     "th_batch_get_x": (C.c_int, [_vp, C.c_int, _dp]),
-    "th_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
-    "th_batch_shape": (C.c_int, [_vp, _ip, _ip]),
+    "th_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    "th_batch_shape": (C.c_int, [_vp, _ip, _ip, _ip]),
     "th_surrogate_param_count": (C.c_long, []),
     "th_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
     "th_surrogate_destroy": (None, [_vp]),
@@ -781,11 +781,11 @@ class Batch:
         return x
 
     def autotune(self, reps=5):
-        r, v, ms = C.c_int(), C.c_int(), C.c_double()
-        _check(lib().th_batch_autotune(self.h, reps, C.byref(r), C.byref(v), C.byref(ms)))
-        return dict(rows=r.value, vec_threads=v.value, vcycle_ms=ms.value)
+        r, c, v, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _check(lib().th_batch_autotune(self.h, reps, C.byref(r), C.byref(c), C.byref(v), C.byref(ms)))
+        return dict(rows=r.value, cols_per_wave=c.value, vec_threads=v.value, vcycle_ms=ms.value)
 
     def shape(self):
-        r, v = C.c_int(), C.c_int()
-        _check(lib().th_batch_shape(self.h, C.byref(r), C.byref(v)))
-        return r.value, v.value
+        r, c, v = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().th_batch_shape(self.h, C.byref(r), C.byref(c), C.byref(v)))
+        return r.value, c.value, v.value

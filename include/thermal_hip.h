@@ -323,11 +323,13 @@ TH_API int th_batch_set_power(th_batch* b, int i, const double* P);
 TH_API int th_batch_solve(th_batch* b, double tol, int maxit, int* iters, double* relres, int* converged,
                           double* ms);
 TH_API int th_batch_get_x(th_batch* b, int i, double* x);
-/* Autotuner: times the V-cycle with 1, 2, 4 or 8 rows of 64 columns per
- * block and the vector stages with 256- or 1024-thread blocks on this batch,
- * keeps the fastest; reports the choice and the V-cycle time. */
-TH_API int th_batch_autotune(th_batch* b, int reps, int* rows, int* vec_threads, double* vcycle_ms);
-TH_API int th_batch_shape(th_batch* b, int* rows, int* vec_threads);
+/* Autotuner: times the V-cycle with blocks of 1, 2, 4 or 8 wavefronts
+ * (*rows) whose wavefronts each cover 64 columns of one row, 32 columns of
+ * two rows or 16 of four (*cols: columns per wavefront) and the vector stages
+ * with 256- or 1024-thread blocks on this batch, keeps the fastest; reports
+ * the choice and the V-cycle time. */
+TH_API int th_batch_autotune(th_batch* b, int reps, int* rows, int* cols, int* vec_threads, double* vcycle_ms);
+TH_API int th_batch_shape(th_batch* b, int* rows, int* cols, int* vec_threads);
 
 /* ---- learned warm start (no reference counterpart) ------------------------
  * A U-Net surrogate (BF16 MFMA kernels, FP32 accumulation) mapping, on an

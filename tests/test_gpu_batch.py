@@ -64,7 +64,7 @@ def test_autotuned_shapes_give_the_same_answers():
     b = T.Batch(ms)
     tune = b.autotune(3)
     assert tune["rows"] in (1, 2, 4, 8) and tune["vec_threads"] in (256, 1024)
-    assert b.shape() == (tune["rows"], tune["vec_threads"])
+    assert b.shape() == (tune["rows"], tune["cols_per_wave"], tune["vec_threads"])
     r = b.solve(1e-10, 200)
     for i, m in enumerate(ms):
         st, x = single(m, 1e-10)
@@ -86,3 +86,36 @@ def test_small_batched_workload_vs_oracle():
         x = b.get_x(i)
         assert np.abs(x - ref["T"]).max() < 5e-4
         assert O.true_relres(a, x) < 1e-9
+
+
+def test_full_batched_workload():
+    """The benchmark's batched workload itself: 64 instances of 128^2, 256^2
+    or 512^2 x 10 cells (seeded, blocks on with p = 0.6), one autotuned
+    batched solve to 1e-10. Every instance converges with energy balance
+    1e-6 and a true relative residual <= 1e-9; every instance matches a
+    single-instance solve to 1e-10 of its peak rise and takes its iteration
+    count within one; the smallest and the largest instance match the FP64
+    oracle to 5e-4 K."""
+    inst = W.config3_instances(count=64)
+    models = [W.build(spec, n) for n, spec in inst]
+    b = T.Batch(models)
+    tune = b.autotune(2)
+    assert tune["cols_per_wave"] in (16, 32, 64)
+    r = b.solve(1e-10, 200)
+    assert r["converged"].all()
+    sizes = [m.nx for m in models]
+    for i, m in enumerate(models):
+        x = b.get_x(i)
+        P = m.total_power()
+        assert abs(m.heat_out(x) - P) / P < 1e-6, i
+        st, xs = single(m, 1e-10)
+        assert abs(int(r["iters"][i]) - st["iters"]) <= 1, (i, r["iters"][i], st["iters"])
+        assert np.abs(x - xs).max() <= 1e-10 * np.abs(xs).max() + 1e-12, i
+    for i in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        a = models[i].arrays()
+        x = b.get_x(i)
+        ref = O.steady(a, tol=1e-10, mg_double=True)
+        assert np.abs(x - ref["T"]).max() < 5e-4
+        assert O.true_relres_flux(a, x) <= 1e-9
+    print(f"64 instances: autotune {tune}, batch {r['ms']:.1f} ms, iterations p50 "
+          f"{np.percentile(r['iters'], 50):.0f} p99 {np.percentile(r['iters'], 99):.0f}")
