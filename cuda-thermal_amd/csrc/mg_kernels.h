@@ -26,7 +26,6 @@ constexpr int kMaxTail = 8;
 constexpr int kTailCols = 32 * 32;  // levels with at most this many columns go to the L2 tail
 constexpr size_t kTailLdsBytes = 148 * 1024;  // LDS budget of the LDS-resident tail (+ 5 or 10 KB static)
 constexpr int kTailLdsThreads = 1024;           // plane-parallel sweeps: 64 / LP columns per wave
-constexpr int kTailScratch = kTailLdsThreads + 64;  // >= LP * (kTailLdsThreads / LP + 1) for LP in {4, 8, 16}
 
 // Coarse parents of the 4 lateral neighbours of column (i, j) (SW_PROLONG).
 struct Parents {
@@ -766,9 +765,9 @@ inline size_t tail_lds_bytes(const Geo* g, int n, int lp) {
 }
 
 // The same tail with every level copied into LDS first (lateral
-// restriction), every sweep plane-parallel: LP lanes per column gather the
-// planes' right-hand sides at once, the column's first lane runs the Thomas
-// recurrence (sweep_column's arithmetic), every lane writes its plane back.
+// restriction), every sweep plane-parallel: the lanes of a column gather the
+// planes' right-hand sides at once, one thread per column runs the Thomas
+// recurrence (sweep_column's arithmetic), every lane writes its planes back.
 // Level arrays are addressed from the dynamic LDS base, so every access is
 // an LDS access. Only the top level's x goes back to global memory, for the
 // prolongation into the level above.
@@ -777,39 +776,56 @@ struct LdsLevel {
     Geo g;
     T *gE, *gN, *gU, *fi, *x, *b;
 };
+// The level whose arrays (gE, gN, gU, 1/pivot, x, b, each g.cells() long)
+// start `off` values into the dynamic LDS.
 template <class T>
-__device__ __forceinline__ LdsLevel<T> lds_level(const Tail<T>& t, T* sm, int k, int lp) {
-    size_t off = 0;
-    for (int m = 0; m < k; ++m) off += 6 * (size_t)tail_lds_geo(t.L[m].g, lp).cells();
-    const Geo g = tail_lds_geo(t.L[k].g, lp);
+__device__ __forceinline__ LdsLevel<T> lds_level_at(const Geo& g, T* sm, long off) {
     const long cells = g.cells();
     T* p = sm + off;
     return LdsLevel<T>{g, p, p + cells, p + 2 * cells, p + 3 * cells, p + 4 * cells, p + 5 * cells};
 }
 
-template <class T, int LP, int MODE>
+// Values in the sweeps' scratch: LP rows (one per plane) of a pass's columns
+// plus one. FP32 levels with more columns of a colour than one pass of LP
+// lanes per column holds run with LP / 2 lanes per column (two planes each),
+// so that a 16 x 16 level is one pass: 2048 + LP values.
+template <class T>
+constexpr int tail_scratch() {
+    return sizeof(T) == 4 ? 2 * kTailLdsThreads + 16 : kTailLdsThreads + 64;
+}
+
+// One colour sweep of an LDS level with LPV lanes per column: lane l0 of a
+// column gathers planes l0, l0 + LPV, ... (LP / LPV of them), so a pass
+// covers kTailLdsThreads / LPV columns; thread k of the pass then solves
+// column k from the scratch.
+template <class T, int LP, int LPV, int MODE>
 __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s,
                                               int* scr_c) {
-    constexpr int CPP = kTailLdsThreads / LP;  // columns per pass
-    constexpr int RS = CPP + 1;                 // scratch row stride (rows by plane, see sweep_pp_body)
-    const int tid = threadIdx.x, l = tid % LP, cp = tid / LP;
+    constexpr int CPP = kTailLdsThreads / LPV;  // columns per pass
+    constexpr int RS = CPP + 1;                  // scratch row stride (rows by plane, see sweep_pp_body)
+    constexpr int PPL = LP / LPV;                // planes per lane
+    static_assert(LP * RS <= tail_scratch<T>() && CPP <= kTailLdsThreads / 4, "tail scratch too small");
+    const int tid = threadIdx.x, l0 = tid % LPV, cp = tid / LPV;
     const Geo& g = F.g;
     const int ncols = g.hx * g.ny;
     for (int q0 = 0; q0 < ncols; q0 += CPP) {
-        const int q = q0 + tid / LP;
+        const int q = q0 + cp;
         const int j = q / g.hx, t = q - j * g.hx;
         const int i = 2 * t + ((j + colour) & 1);
         const bool colok = q < ncols && i < g.nx;
-        const bool ok = colok && l < g.nz;
         Col c{};
-        if (colok) c = make_col(g, i, j);
-        if (ok) {
+        if (colok) {
+            c = make_col(g, i, j);
             const Parents pa = MODE == SW_PROLONG ? parents(C.g, c, i, j) : Parents{0, 0, 0, 0};
-            scr_s[l * RS + cp] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
+#pragma unroll
+            for (int r = 0; r < PPL; ++r) {
+                const int l = l0 + r * LPV;
+                if (l < g.nz) scr_s[l * RS + cp] = column_rhs<T, MODE>(g, F.gE, F.gN, F.b, F.x, c, l, C.g, C.x, pa);
+            }
         }
-        if (l == 0) scr_c[cp] = colok ? c.c0 : -1;
+        if (l0 == 0) scr_c[cp] = colok ? c.c0 : -1;
         __syncthreads();
-        // Thread k < CPP solves column k of the pass (one wave at LP = 16).
+        // Thread k < CPP solves column k of the pass (one wave at LPV = 16).
         const int c0 = tid < CPP ? scr_c[tid] : -1;
         if (c0 >= 0) {
             T* sc = scr_s + tid;  // plane k at sc[k * RS]
@@ -846,69 +862,131 @@ __device__ __forceinline__ void tail_sweep_pp(const LdsLevel<T>& F, int colour, 
             }
         }
         __syncthreads();
-        if (ok) F.x[(long)l * g.plane + c.c0] = scr_s[l * RS + cp];
+        if (colok) {
+#pragma unroll
+            for (int r = 0; r < PPL; ++r) {
+                const int l = l0 + r * LPV;
+                if (l < g.nz) F.x[(long)l * g.plane + c.c0] = scr_s[l * RS + cp];
+            }
+        }
     }
     __syncthreads();
 }
+// A sweep in one pass where two planes per lane make it one (FP32, LP >= 8).
+// At 16 x 16 x 16 one pass instead of two takes each sweep from ~3.6 to ~2.4 us.
+template <class T, int LP, int MODE>
+__device__ __forceinline__ void tail_sweep_lds(const LdsLevel<T>& F, int colour, const LdsLevel<T>& C, T* scr_s,
+                                               int* scr_c) {
+    if constexpr (sizeof(T) == 4 && LP >= 8) {
+        if (F.g.hx * F.g.ny > kTailLdsThreads / LP) {
+            tail_sweep_pp<T, LP, LP / 2, MODE>(F, colour, C, scr_s, scr_c);
+            return;
+        }
+    }
+    tail_sweep_pp<T, LP, LP, MODE>(F, colour, C, scr_s, scr_c);
+}
 
+// The levels' LDS descriptors are walked level to level (the next one from
+// the current one's offset and one geometry read from the tail argument):
+// rebuilding each from level 0 read every level's geometry in a chain of
+// dependent scalar loads, ~0.8 us per level visit. Staging works in 32-bit
+// indices (the tail's levels are small) with a thread's cells of a level
+// loaded before any is stored. 16^2 .. 1^2 x 16: 55 -> 47 us per launch,
+// results unchanged bit for bit.
 template <class T, int NZ>
 __device__ __forceinline__ void tail_lds_body(const Tail<T>& t, State* st, const double* __restrict__ cons_part,
                                               int cons_nblk, int cons_mode, T* sm) {
     constexpr int LP = pp_lanes(NZ);
-    __shared__ __align__(16) T scr_s[kTailScratch];
+    __shared__ __align__(16) T scr_s[tail_scratch<T>()];
     __shared__ int scr_c[kTailLdsThreads / 4];  // pass columns' c0 (-1: none)
-    if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
-    for (int k = 0; k < t.n; ++k) {
-        const TailLevel<T>& G = t.L[k];
-        const LdsLevel<T> D = lds_level<T>(t, sm, k, LP);
-        const long cells = G.g.cells();
-        for (long q = threadIdx.x; q < cells; q += kTailLdsThreads) {
-            const long l = q / G.g.plane, sl = q - l * G.g.plane, d = l * D.g.plane + sl;
-            if (sl >= D.g.slots()) continue;  // pad slots are never read
-            D.gE[d] = G.o.gE[q];
-            D.gN[d] = G.o.gN[q];
-            D.gU[d] = G.o.gU[q];
-            D.fi[d] = G.finv[q];
-            if (k == 0) D.b[d] = G.b[q];
+    const int done = st->done;  // in flight with the first level's loads when no test is carried
+    if (cons_mode != CONS_NONE) {
+        if (done) return;
+        if (consume_convergence(cons_part, cons_nblk, cons_mode, st, true)) return;
+    }
+    {
+        long off = 0;
+        for (int k = 0; k < t.n; ++k) {
+            const TailLevel<T>& G = t.L[k];
+            const Geo dg = tail_lds_geo(G.g, LP);
+            const int cells = (int)G.g.cells(), gp = (int)G.g.plane, dp = (int)dg.plane, slots = (int)dg.slots();
+            const int dc = (int)dg.cells();
+            T* p = sm + off;
+            constexpr int SPT = 4;
+            for (int base = 0; base < cells; base += SPT * kTailLdsThreads) {
+                T vE[SPT], vN[SPT], vU[SPT], vF[SPT], vB[SPT];
+                int dd[SPT];
+#pragma unroll
+                for (int r = 0; r < SPT; ++r) {
+                    const int q = base + (int)threadIdx.x + r * kTailLdsThreads;
+                    dd[r] = -1;
+                    if (q < cells) {
+                        const int l = q / gp, sl = q - l * gp;
+                        if (sl < slots) dd[r] = l * dp + sl;  // pad slots are never read
+                        vE[r] = G.o.gE[q];
+                        vN[r] = G.o.gN[q];
+                        vU[r] = G.o.gU[q];
+                        vF[r] = G.finv[q];
+                        vB[r] = k == 0 ? G.b[q] : T(0);
+                    }
+                }
+                if (done) return;
+#pragma unroll
+                for (int r = 0; r < SPT; ++r) {
+                    if (dd[r] < 0) continue;
+                    p[dd[r]] = vE[r];
+                    p[dc + dd[r]] = vN[r];
+                    p[2 * dc + dd[r]] = vU[r];
+                    p[3 * dc + dd[r]] = vF[r];
+                    if (k == 0) p[5 * dc + dd[r]] = vB[r];
+                }
+            }
+            off += 6L * dc;
         }
     }
     __syncthreads();
     const int n = t.n;
+    long offF = 0;
+    LdsLevel<T> F = lds_level_at<T>(tail_lds_geo(t.L[0].g, LP), sm, 0);
     for (int k = 0; k + 1 < n; ++k) {
-        const LdsLevel<T> F = lds_level<T>(t, sm, k, LP), C = lds_level<T>(t, sm, k + 1, LP);
-        tail_sweep_pp<T, LP, SW_ZERO>(F, 0, C, scr_s, scr_c);
-        tail_sweep_pp<T, LP, SW_PLAIN>(F, 1, C, scr_s, scr_c);
+        const long offC = offF + 6 * F.g.cells();
+        const LdsLevel<T> C = lds_level_at<T>(tail_lds_geo(t.L[k + 1].g, LP), sm, offC);
+        tail_sweep_lds<T, LP, SW_ZERO>(F, 0, C, scr_s, scr_c);
+        tail_sweep_lds<T, LP, SW_PLAIN>(F, 1, C, scr_s, scr_c);
         const int cells = C.g.nx * C.g.ny * C.g.nz;
         for (int q = threadIdx.x; q < cells; q += kTailLdsThreads) {
             const int I = q % C.g.nx, rest = q / C.g.nx;
             restrict_cell_fast<T>(F.g, F.gE, F.gN, F.x, C.g, C.b, I, rest % C.g.ny, rest / C.g.ny);
         }
         __syncthreads();
+        F = C;
+        offF = offC;
     }
-    {
-        const LdsLevel<T> N = lds_level<T>(t, sm, n - 1, LP);
-        tail_sweep_pp<T, LP, SW_ZERO>(N, 0, N, scr_s, scr_c);
-    }
+    tail_sweep_lds<T, LP, SW_ZERO>(F, 0, F, scr_s, scr_c);
+    LdsLevel<T> C = F;
+    long offC = offF;
     for (int k = n - 2; k >= 0; --k) {
-        const LdsLevel<T> F = lds_level<T>(t, sm, k, LP), C = lds_level<T>(t, sm, k + 1, LP);
-        tail_sweep_pp<T, LP, SW_PROLONG>(F, 1, C, scr_s, scr_c);
-        tail_sweep_pp<T, LP, SW_PLAIN>(F, 0, C, scr_s, scr_c);
+        const Geo fg = tail_lds_geo(t.L[k].g, LP);
+        offF = offC - 6 * fg.cells();
+        F = lds_level_at<T>(fg, sm, offF);
+        tail_sweep_lds<T, LP, SW_PROLONG>(F, 1, C, scr_s, scr_c);
+        tail_sweep_lds<T, LP, SW_PLAIN>(F, 0, C, scr_s, scr_c);
+        C = F;
+        offC = offF;
     }
-    // Only slots of real columns come back: a padding slot (odd nx: the
-    // half-row's last slot in the rows whose colour has no column there) is
-    // never written in LDS, so it holds whatever an earlier workgroup left
-    // there, and it must stay 0 in global memory: the PCG's r.z and
-    // r.z_old run over every slot, and r = 0 there times an Inf or NaN left
-    // in LDS would make them NaN.
-    const LdsLevel<T> T0 = lds_level<T>(t, sm, 0, LP);
+    // F is level 0 again. Only slots of real columns come back: a padding
+    // slot (odd nx: the half-row's last slot in the rows whose colour has no
+    // column there) is never written in LDS, so it holds whatever an earlier
+    // workgroup left there, and it must stay 0 in global memory: the PCG's
+    // r.z and r.z_old run over every slot, and r = 0 there times an Inf or
+    // NaN left in LDS would make them NaN.
     const Geo G0 = t.L[0].g;
-    const long cells0 = G0.cells();
-    for (long q = threadIdx.x; q < cells0; q += kTailLdsThreads) {
-        const long l = q / G0.plane, sl = q - l * G0.plane;
-        const int row = (int)(sl / G0.hx), ts = (int)(sl - (long)row * G0.hx);
-        const bool real = sl < G0.slots() && 2 * ts + (((row >> 1) + (row & 1)) & 1) < G0.nx;
-        t.L[0].x[q] = real ? T0.x[l * T0.g.plane + sl] : T(0);
+    const int cells0 = (int)G0.cells(), gp0 = (int)G0.plane, fp0 = (int)F.g.plane, slots0 = (int)G0.slots();
+    for (int q = threadIdx.x; q < cells0; q += kTailLdsThreads) {
+        const int l = q / gp0, sl = q - l * gp0;
+        const int row = sl / G0.hx, ts = sl - row * G0.hx;
+        const bool real = sl < slots0 && 2 * ts + (((row >> 1) + (row & 1)) & 1) < G0.nx;
+        t.L[0].x[q] = real ? F.x[l * fp0 + sl] : T(0);
     }
 }
 template <class T, int NZ>
