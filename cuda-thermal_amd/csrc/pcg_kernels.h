@@ -876,28 +876,39 @@ struct LiftMap {
 
 // x(i, j, l) = scale * the coarse field at the centre of cell (i, j),
 // bilinear over the Hc x Wc coarse cells (clamped at the edges), then
-// linear in z through the map.
-static __global__ void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C, LiftMap zm, double scale,
-                       double* __restrict__ x) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long)g.nx * g.ny * g.nz) return;
-    const int i = (int)(k % g.nx);
-    const long rest = k / g.nx;
-    const int j = (int)(rest % g.ny), l = (int)(rest / g.ny);
+// linear in z through the map. One thread per column (a slot of a row, both
+// colours): the bilinear weights are formed once per column and a wave
+// stores each plane as one run of a half-row; the same arithmetic per cell
+// as one thread per cell, so the same values. 1024^2 x 16 from 256^2: 99 ->
+// 86 us. (Issuing all planes' loads before the first store took it to 344
+// us: the gathers of a wave touch ~32 coarse pixels each, and 128 of them in
+// flight per lane thrash the L1.)
+static __global__ __launch_bounds__(256) void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C,
+                                                     LiftMap zm, double scale, double* __restrict__ x) {
+    const int q = blockIdx.x * kColsX + threadIdx.x;
+    const int j = blockIdx.y * kColsY + threadIdx.y;
+    int i;
+    if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
     const double xc = fmin(fmax((i + 0.5) * Wc / g.nx - 0.5, 0.0), (double)(Wc - 1));
     const double yc = fmin(fmax((j + 0.5) * Hc / g.ny - 0.5, 0.0), (double)(Hc - 1));
     const int x0 = (int)xc, y0 = (int)yc;
     const int x1 = x0 + 1 < Wc ? x0 + 1 : x0, y1 = y0 + 1 < Hc ? y0 + 1 : y0;
     const double fx = xc - x0, fy = yc - y0;
-    const int lo = zm.lo[l];
-    const double w = zm.w[l];
-    auto at = [&](int yy, int xx) {
-        const float* p = f + ((long)yy * Wc + xx) * C + lo;
-        return w > 0.0 ? (1.0 - w) * p[0] + w * p[1] : (double)p[0];
-    };
-    const double v = (1.0 - fy) * ((1.0 - fx) * at(y0, x0) + fx * at(y0, x1)) +
-                     fy * ((1.0 - fx) * at(y1, x0) + fx * at(y1, x1));
-    x[(long)l * g.plane + g.pos(i, j)] = scale * v;
+    const float* r00 = f + ((long)y0 * Wc + x0) * C;
+    const float* r01 = f + ((long)y0 * Wc + x1) * C;
+    const float* r10 = f + ((long)y1 * Wc + x0) * C;
+    const float* r11 = f + ((long)y1 * Wc + x1) * C;
+    const long c0 = g.pos(i, j);
+    for (int l = 0; l < g.nz; ++l) {
+        const int lo = zm.lo[l];
+        const double w = zm.w[l];
+        auto at = [&](const float* r) {
+            const float* p = r + lo;
+            return w > 0.0 ? (1.0 - w) * p[0] + w * p[1] : (double)p[0];
+        };
+        const double v = (1.0 - fy) * ((1.0 - fx) * at(r00) + fx * at(r01)) + fy * ((1.0 - fx) * at(r10) + fx * at(r11));
+        x[(long)l * g.plane + c0] = scale * v;
+    }
 }
 
 // One colour of a column line relaxation of the steady operator in FP64:

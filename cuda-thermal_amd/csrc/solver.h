@@ -1013,7 +1013,7 @@ struct Solver final : SolverBase {
     // v = the coarse field lifted to the fine grid, then `passes` red-black
     // column line relaxations of the steady operator.
     int lift_into(double* v, const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes) {
-        k_lift<<<lin_blocks(nnat), 256, 0, s>>>(g0, field, Hc, Wc, C, zm, scale, v);
+        k_lift<<<col_grid(2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, field, Hc, Wc, C, zm, scale, v);
         HIPTRY(hipGetLastError());
         const Ops<T> o = L[0].ops();
         for (int k = 0; k < passes; ++k)
