@@ -636,37 +636,65 @@ __global__ __launch_bounds__(kPPThreads) void k_restrict_zero_pp(Geo f, Ops<T> f
     if (ok && colour == 0) x[(long)l * g.plane + g.pos(i, j)] = sh_s[l * RS + cb];
 }
 
-// restrict_cell_fast with the red children's faces from their types.
+// restrict_cell_fast with the red children's faces from their types, for
+// kRestrictPlanes planes per thread (grid z: the plane groups): every x of
+// the thread's planes is loaded before the first type row is read. At
+// 1024^2 x 16 against one plane per thread and the loads after the type:
+// 20.4 -> 13.7 us back to back, 33.9 -> 26.5 us with the caches flushed; the
+// same sums, so the same values.
+constexpr int kRestrictPlanes = 2;
 template <class T>
 __global__ __launch_bounds__(256) void k_restrict_fast_typed(Geo f, Typed<T> y, const T* __restrict__ fx, Geo cg,
                                                              T* __restrict__ cb, const State* __restrict__ st) {
+    constexpr int P = kRestrictPlanes;
     if (st->done) return;
     const int I = blockIdx.x * kColsX + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= cg.nx || J >= cg.ny) return;
-    const int l = blockIdx.z;
-    const long base = (long)l * f.plane;
     const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
-    T r[2];
+    Col c[2];
+    int t[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        const Col c = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
-        const int t = y.ty[c.c0];
-        const TypeRow<T> w = y.row(t, wave_rep_type(t), l);
-        const T Gw = c.hw ? w.v[TF_W] : T(0), Ge = c.he ? w.v[TF_E] : T(0);
-        const T Gs = c.hs ? w.v[TF_S] : T(0), Gn = c.hn ? w.v[TF_N] : T(0);
-        const T xw = fx[base + c.pw], xe = fx[base + c.pe], xs = fx[base + c.ps], xn = fx[base + c.pn];
-        T s = 0;  // absent faces add (+0) * x, as in sweep_column_typed
-        s += Gw * xw;
-        s += Ge * xe;
-        s += Gs * xs;
-        s += Gn * xn;
-        r[k] = s;
+        c[k] = make_col(f, 2 * I + (two ? k : 0), 2 * J + (two ? k : 0));
+        t[k] = y.ty[c[k].c0];
     }
-    T sum = 0;
-    sum += r[0];
-    if (two) sum += r[1];
-    cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+    const int l0 = blockIdx.z * P;
+    T xv[P][2][4];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int l = l0 + p < f.nz ? l0 + p : f.nz - 1;  // past nz: the top plane again, unused
+        const long base = (long)l * f.plane;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            xv[p][k][0] = fx[base + c[k].pw];
+            xv[p][k][1] = fx[base + c[k].pe];
+            xv[p][k][2] = fx[base + c[k].ps];
+            xv[p][k][3] = fx[base + c[k].pn];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int l = l0 + p;
+        if (l >= f.nz) break;
+        T r[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const TypeRow<T> w = y.row(t[k], wave_rep_type(t[k]), l);
+            const T Gw = c[k].hw ? w.v[TF_W] : T(0), Ge = c[k].he ? w.v[TF_E] : T(0);
+            const T Gs = c[k].hs ? w.v[TF_S] : T(0), Gn = c[k].hn ? w.v[TF_N] : T(0);
+            T s = 0;  // absent faces add (+0) * x, as in sweep_column_typed
+            s += Gw * xv[p][k][0];
+            s += Ge * xv[p][k][1];
+            s += Gs * xv[p][k][2];
+            s += Gn * xv[p][k][3];
+            r[k] = s;
+        }
+        T sum = 0;
+        sum += r[0];
+        if (two) sum += r[1];
+        cb[(long)l * cg.plane + cg.pos(I, J)] = sum;
+    }
 }
 
 // ---- coarse tail: the lowest levels of the V-cycle in one workgroup --------

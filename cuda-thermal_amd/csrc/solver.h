@@ -580,9 +580,10 @@ struct Solver final : SolverBase {
         const DevLevel<T>& c = L[k + 1];
         dim3 grid = col_grid(c.g.nx, c.g.ny);
         grid.z = c.g.nz;
-        if (fast && f.ntypes)
+        if (fast && f.ntypes) {
+            grid.z = (c.g.nz + kRestrictPlanes - 1) / kRestrictPlanes;
             k_restrict_fast_typed<T><<<grid, col_block(), 0, s>>>(f.g, f.typed(), f.x, c.g, c.b, st);
-        else if (fast)
+        } else if (fast)
             k_restrict_fast<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.x, c.g, c.b, st);
         else
             k_restrict<T><<<grid, col_block(), 0, s>>>(f.g, f.ops(), f.b, f.x, c.g, c.b, st);
