@@ -186,77 +186,128 @@ __global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf
     }
 }
 
-// Partial weight gradients of one pixel chunk for one tap:
-// part[chunk][tap][cout][cin] = sum_p GZ[p][co] X[p + off(tap)][ci]; tap ==
-// KS*KS is the bias (input 1 in channel 0).
+// Partial weight gradients of one pixel chunk: part[chunk][tap][cout][cin]
+// = sum_p GZ[p][co] X[p + off(tap)][ci] (zero outside the image); tap ==
+// KS*KS is the bias, sum_p GZ[p][co], in column ci = 0. A GEMM over pixels
+// per tap (M = COUT, N = CIN, K = pixels) whose operands are pixel-major in
+// memory, while an MFMA lane takes 8 consecutive K values of one row or
+// column: the block stages each K-block of 32 pixels transposed in LDS,
+// GZ^T [COUT][32] once and X^T [CIN][32] per tap (the tap's shifted pixels),
+// so that each operand fragment is one 16-byte LDS read, and issues
+// v_mfma_f32_16x16x32_bf16 on them; wave w < KS*KS accumulates tap w's
+// COUT x CIN tile set, the last wave the bias (B = 1 in column 0). Staging
+// lanes run along the 32 pixels of a K-block, so a row of the transposed
+// tile is written by consecutive lanes. The chunk is a multiple of 32
+// pixels.
 template <int KS, int CIN, int COUT>
-__global__ __launch_bounds__(64) void k_wgrad(const bf* __restrict__ X, const bf* __restrict__ GZ,
-                                              float* __restrict__ part, long npix, int H, int W, long chunk) {
-    constexpr int KK = KS * KS, NT = COUT / 16, NU = CIN / 16;
-    const int lane = threadIdx.x, m = lane & 15, q = lane >> 4;
-    const int tap = blockIdx.y;
-    const int dy = tap / KS - KS / 2, dx = tap % KS - KS / 2;
-    const long pb = (long)blockIdx.x * chunk, pe = (pb + chunk < npix) ? pb + chunk : npix;
-    const long hw = (long)H * W;
+__global__ __launch_bounds__((KS * KS + 1) * 64) void k_wgrad(const bf* __restrict__ X, const bf* __restrict__ GZ,
+                                                              float* __restrict__ part, int npix, int H, int W,
+                                                              int chunk) {
+    constexpr int KK = KS * KS, NWAVE = KK + 1, NT = COUT / 16, NU = CIN / 16, KB = 32;
+    constexpr int RS = KB + 8;  // row stride in BF16 values: 16-byte aligned rows
+    constexpr int NTH = NWAVE * 64;
+    __shared__ __align__(16) bf sg[COUT * RS];
+    __shared__ __align__(16) bf sx[KK * CIN * RS];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, m = lane & 15, q = lane >> 4;
+    const int pb = blockIdx.x * chunk, pe = pb + chunk < npix ? pb + chunk : npix;
+    const int hw = H * W;
     f4v acc[NT][NU];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int u = 0; u < NU; ++u) acc[t][u] = f4v{0.f, 0.f, 0.f, 0.f};
-    for (long p0 = pb; p0 < pe; p0 += 16) {
-        s4v a[NT], b[NU];
+    for (int p0 = pb; p0 < pe; p0 += KB) {
+        // GZ^T: element e -> pixel k = e % 32, channel group g = e / 32 (8 channels)
+        for (int e = tid; e < KB * (COUT / 8); e += NTH) {
+            const int k = e % KB, g = e / KB, p = p0 + k;
+            u4v v = p < pe ? *(const u4v*)(GZ + (long)p * COUT + 8 * g) : u4v{0u, 0u, 0u, 0u};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long p = p0 + 4 * q + j;
-            const bool ok = p < pe;
-#pragma unroll
-            for (int t = 0; t < NT; ++t) a[t][j] = ok ? (short)GZ[p * COUT + t * 16 + m] : (short)0;
-            if (tap == KK) {
-#pragma unroll
-                for (int u = 0; u < NU; ++u) b[u][j] = (ok && u == 0 && m == 0) ? (short)0x3F80 : (short)0;
-            } else {
-                const long img = ok ? p / hw : 0;
-                const int rem = ok ? (int)(p - img * hw) : 0;
-                const int yy = rem / W + dy, xx = rem % W + dx;
-                const bool inb = ok && yy >= 0 && yy < H && xx >= 0 && xx < W;
-                const bf* xp = X + (img * hw + (long)yy * W + xx) * CIN + m;
-#pragma unroll
-                for (int u = 0; u < NU; ++u) b[u][j] = inb ? (short)xp[u * 16] : (short)0;
-            }
+            for (int c = 0; c < 8; ++c) sg[(8 * g + c) * RS + k] = (bf)((v[c >> 1] >> (16 * (c & 1))) & 0xFFFFu);
         }
+        // X^T of every tap: element e -> tap, pixel k, channel group g
+        for (int e = tid; e < KK * KB * (CIN / 8); e += NTH) {
+            const int k = e % KB, r = e / KB, g = r % (CIN / 8), tap = r / (CIN / 8), p = p0 + k;
+            u4v v{0u, 0u, 0u, 0u};
+            if (p < pe) {
+                const int img = p / hw, rem = p - img * hw, y = rem / W, x = rem - y * W;
+                const int yy = y + tap / KS - KS / 2, xx = x + tap % KS - KS / 2;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = *(const u4v*)(X + ((long)img * hw + (long)yy * W + xx) * CIN + 8 * g);
+            }
+            bf* d = sx + (tap * CIN + 8 * g) * RS + k;
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+            for (int c = 0; c < 8; ++c) d[c * RS] = (bf)((v[c >> 1] >> (16 * (c & 1))) & 0xFFFFu);
+        }
+        __syncthreads();
+        b8v a[NT];
 #pragma unroll
-            for (int u = 0; u < NU; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b[u], acc[t][u], 0, 0, 0);
+        for (int t = 0; t < NT; ++t) a[t] = *(const b8v*)(sg + (16 * t + m) * RS + 8 * q);
+        if (wave < KK) {
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const b8v bx = *(const b8v*)(sx + (wave * CIN + 16 * u + m) * RS + 8 * q);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bx, acc[t][u], 0, 0, 0);
+            }
+        } else {
+            const unsigned one = m == 0 ? 0x3F803F80u : 0u;  // BF16 1.0 pairs in column 0
+            const b8v ones = __builtin_bit_cast(b8v, u4v{one, one, one, one});
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], ones, acc[t][0], 0, 0, 0);
+        }
+        __syncthreads();
     }
-    float* out = part + ((long)blockIdx.x * (KK + 1) + tap) * COUT * CIN;
+    // lane (m, q) holds D[co = 16 t + 4 q + i][ci = 16 u + m]
+    float* out = part + ((long)blockIdx.x * (KK + 1) + wave) * COUT * CIN;
+    const int nu = wave < KK ? NU : 1;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int u = 0; u < NU; ++u)
+            if (u < nu)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) out[(t * 16 + 4 * q + i) * CIN + u * 16 + m] = acc[t][u][i];
+                for (int i = 0; i < 4; ++i) out[(16 * t + 4 * q + i) * CIN + 16 * u + m] = acc[t][u][i];
 }
 
 // Sums the chunks' partials into the canonical gradient layout
-// gW[cout][tap][cin], gb[cout].
-__global__ void k_reduce_grad(const float* __restrict__ part, int nchunks, int KK, int COUT, int CIN,
-                              float* __restrict__ gW, float* __restrict__ gb) {
+// gW[cout][tap][cin], gb[cout]. A block takes 64 gradient values; its 4
+// waves each sum every 4th chunk (4 loads in flight per thread), then the
+// four partial sums are added in a fixed order: the result does not depend
+// on the launch.
+__global__ __launch_bounds__(256) void k_reduce_grad(const float* __restrict__ part, int nchunks, int KK, int COUT,
+                                                     int CIN, float* __restrict__ gW, float* __restrict__ gb) {
+    __shared__ float sh[4][64];
     const int n = COUT * KK * CIN + COUT;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    long src;
-    if (k < COUT * KK * CIN) {
-        const int co = k / (KK * CIN), tap = (k / CIN) % KK, ci = k % CIN;
-        src = ((long)tap * COUT + co) * CIN + ci;
-    } else {
-        src = ((long)KK * COUT + (k - COUT * KK * CIN)) * CIN;
+    const int e = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + e;
+    long src = 0;
+    if (k < n) {
+        if (k < COUT * KK * CIN) {
+            const int co = k / (KK * CIN), tap = (k / CIN) % KK, ci = k % CIN;
+            src = ((long)tap * COUT + co) * CIN + ci;
+        } else {
+            src = ((long)KK * COUT + (k - COUT * KK * CIN)) * CIN;
+        }
     }
     const long stride = (long)(KK + 1) * COUT * CIN;
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += part[c * stride + src];
-    if (k < COUT * KK * CIN) gW[k] = s;
-    else gb[k - COUT * KK * CIN] = s;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (k < n) {
+        int c = w;
+        for (; c + 12 < nchunks; c += 16) {
+            s0 += part[c * stride + src];
+            s1 += part[(c + 4) * stride + src];
+            s2 += part[(c + 8) * stride + src];
+            s3 += part[(c + 12) * stride + src];
+        }
+        for (; c < nchunks; c += 4) s0 += part[c * stride + src];
+    }
+    sh[w][e] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (w == 0 && k < n) {
+        const float s = ((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e];
+        if (k < COUT * KK * CIN) gW[k] = s;
+        else gb[k - COUT * KK * CIN] = s;
+    }
 }
 
 // Elementwise kernels over NHWC tensors, 8 channels (16 bytes) per thread:
@@ -374,14 +425,16 @@ __global__ void k_pack_input(const float* __restrict__ in, bf* __restrict__ X0, 
 }
 
 // MSE over the 10 used outputs: gradient 2 (out - tgt) / count in BF16 and
-// per-block partial sums of the squared error. idx == nullptr: targets are
-// batch-contiguous.
+// per-block partial sums of the squared error (a grid-stride loop, so the
+// partials number the blocks, not the elements). idx == nullptr: targets
+// are batch-contiguous.
+constexpr int kMseBlocks = 1024;
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, const float* __restrict__ tgt,
                                              const int* __restrict__ idx, long hw, long npix, double inv_cnt,
                                              bf* __restrict__ g, double* __restrict__ part) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
-    if (k < npix * 16) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < npix * 16; k += stride) {
         const int c = (int)(k & 15);
         const long pix = k >> 4;
         float gv = 0.f;
@@ -389,7 +442,7 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, cons
             const long img = pix / hw, p = pix - img * hw;
             const long t = ((idx ? (long)idx[img] : img) * hw + p) * kOut + c;
             const float d = out[k] - tgt[t];
-            e = (double)d * d;
+            e += (double)d * d;
             gv = (float)(2.0 * d * inv_cnt);
         }
         g[k] = f2bf(gv);
@@ -668,10 +721,15 @@ struct Net {
         const long n = npix(l.level, B);
         const int h = H >> l.level, w = W >> l.level;
         const int KK = l.ks * l.ks;
-        const int nch = (int)std::max(1L, std::min((long)kMaxChunks, (n + 2047) / 2048));
-        const long chunk = (n + nch - 1) / nch;
-        const dim3 g(nch, KK + 1);
-#define SUR_WG(KS, CI, CO) k_wgrad<KS, CI, CO><<<g, 64, 0, s>>>(xin, gz, wpart, n, h, w, chunk)
+        // chunks of a multiple of 32 pixels (the kernel's K-blocks), at most
+        // kMaxChunks and at least 512 pixels each: small layers get enough
+        // blocks, large ones no more partials than the reduction reads fast
+        long chunk = (n + kMaxChunks - 1) / kMaxChunks;
+        chunk = std::max(512L, (chunk + 31) / 32 * 32);
+        const int nch = (int)((n + chunk - 1) / chunk);
+        const int nthr = (KK + 1) * 64;
+#define SUR_WG(KS, CI, CO) \
+    k_wgrad<KS, CI, CO><<<nch, nthr, 0, s>>>(xin, gz, wpart, (int)n, h, w, (int)chunk)
         if (l.ks == 1 && l.cin == 32 && l.cout == 16) SUR_WG(1, 32, 16);
         else if (l.ks == 3 && l.cin == 16 && l.cout == 32) SUR_WG(3, 16, 32);
         else if (l.ks == 3 && l.cin == 32 && l.cout == 32) SUR_WG(3, 32, 32);
@@ -683,7 +741,7 @@ struct Net {
         SURTRY(hipGetLastError());
         float* gW = grad + off[k];
         float* gb = gW + (long)l.cout * KK * l.cin;
-        k_reduce_grad<<<grid1((long)l.cout * KK * l.cin + l.cout), 256, 0, s>>>(wpart, nch, KK, l.cout, l.cin, gW, gb);
+        k_reduce_grad<<<grid1((long)l.cout * KK * l.cin + l.cout, 64), 256, 0, s>>>(wpart, nch, KK, l.cout, l.cin, gW, gb);
         SURTRY(hipGetLastError());
         return TH_OK;
     }
@@ -774,7 +832,7 @@ struct Net {
     // loss of the current outputs into *dst (device), gradient into g_out
     int loss(int B, const float* tgt, const int* idx, double* dst) {
         const long n0 = npix(0, B);
-        const unsigned g = grid1(n0 * 16);
+        const unsigned g = std::min<unsigned>(grid1(n0 * 16), kMseBlocks);
         k_mse<<<g, 256, 0, s>>>(out, tgt, idx, hw(), n0, 1.0 / ((double)n0 * kOut), g_out, lpart);
         SURTRY(hipGetLastError());
         k_sum_to<<<1, 256, 0, s>>>(lpart, (int)g, 1.0 / ((double)n0 * kOut), dst);
