@@ -63,8 +63,12 @@ def test_forward_matches_oracle(n, B):
     assert np.sqrt(np.mean((out - ref) ** 2)) < 3e-3 * scale
 
 
-def test_gradients_match_oracle():
-    n, B = 32, 2
+@pytest.mark.parametrize("n,B", [(32, 2), (48, 3)])
+def test_gradients_match_oracle(n, B):
+    """Loss and every layer's gradient against the oracle. 48 x 48: rows of
+    48, 24, 12, 6 and 3 pixels, so the weight gradient's K-blocks of 32
+    pixels straddle rows and images and the last block of a chunk is
+    partly past the batch."""
     sur = T.Surrogate(n, n, seed=12)
     p = sur.get_params()
     x = features_batch(B, n, 2)
