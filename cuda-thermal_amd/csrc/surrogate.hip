@@ -84,16 +84,23 @@ enum { EPI_RELU = 0, EPI_LINEAR = 1, EPI_F32 = 2 };
 // fragments once and issues MT x COUT / 16 MFMAs, so every weight fragment
 // feeds MT of them (with MT = 1 a weight fragment read through L1 fed one
 // MFMA, and the kernel was bound by those reads, not by the matrix cores).
-// Each output's MFMA chain runs over (tap, K-slice) in the same order for
-// any MT, so the results do not depend on it.
-template <int KS, int CIN, int COUT, int EPI, int MT = 4>
+// On the small levels a wave takes TW of the COUT / 16 output tiles (waves
+// (pixel tile, output group)): with every tile in one wave, its 9 taps'
+// weight loads ran one after another (~14 us per 3x3 64-channel layer at
+// 16 x 16 pixels whatever the size). Each output's MFMA chain runs over
+// (tap, K-slice) in the same order for any MT and TW, so the results do not
+// depend on them.
+template <int KS, int CIN, int COUT, int EPI, int MT = 4, int TW = COUT / 16>
 __global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf* __restrict__ Wf,
                                               const float* __restrict__ bias, void* __restrict__ Yv, long npix,
                                               int H, int W) {
     constexpr int KC = (CIN % 32 == 0) ? 32 : 16;  // K of one MFMA
-    constexpr int NT = COUT / 16;
+    constexpr int NT = TW, NG = COUT / 16 / TW;    // tiles per wave, output groups
+    static_assert(NG * TW * 16 == COUT, "output groups");
     const int lane = threadIdx.x & 63;
-    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * MT);
+    const long wv = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long p0 = (wv / NG) * (16 * MT);
+    const int t0 = (int)(wv % NG) * TW;  // first output tile of the wave
     if (p0 >= npix) return;
     const int m = lane & 15, q = lane >> 4;
     const long hw = (long)H * W;
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf
             inb[u] = valid[u] && yy >= 0 && yy < H && xx >= 0 && xx < W;
             xp[u] = X + (img[u] * hw + (long)(inb[u] ? yy : 0) * W + (inb[u] ? xx : 0)) * CIN;
         }
-        const bf* wp = Wf + ((long)tap * COUT + m) * CIN;
+        const bf* wp = Wf + ((long)tap * COUT + 16 * t0 + m) * CIN;
 #pragma unroll
         for (int c0 = 0; c0 < CIN; c0 += KC) {
             if constexpr (KC == 32) {
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf* __restrict__ X, const bf
         if (pr >= npix) continue;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int co = t * 16 + 4 * q;
+            const int co = (t0 + t) * 16 + 4 * q;
             float v[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -667,17 +674,21 @@ struct Net {
     // four waves; 16 otherwise (a batch of one 256 x 256 image at the top
     // level would give each SIMD a single wave).
     static bool conv_wide(long npix) { return npix >= 64L * 4 * 1024; }
+    // below this many pixels a wave takes one output tile (k_conv's TW = 1)
+    static bool conv_narrow(long npix) { return npix < 65536; }
     int conv(int k, const bf* in, void* outp, int B, int epi) {
         const LayerSpec& l = kSpec[k];
         const long n = npix(l.level, B);
         const int h = H >> l.level, w = W >> l.level;
-        const bool wide = conv_wide(n);
+        const bool wide = conv_wide(n), narrow = conv_narrow(n);
         const unsigned g = grid1(n, wide ? 256 : 64);  // 4 waves of 64 or 16 pixels
+        const unsigned gn = grid1(n * (l.cout / 16), 64);  // 4 waves of 16 pixels x 16 channels
         const float* bias = prm + off[k] + (long)l.cout * l.ks * l.ks * l.cin;
-#define SUR_CONV(KS, CI, CO, EPI)                                                              \
-    do {                                                                                       \
-        if (wide) k_conv<KS, CI, CO, EPI, 4><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w); \
-        else k_conv<KS, CI, CO, EPI, 1><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w);      \
+#define SUR_CONV(KS, CI, CO, EPI)                                                                 \
+    do {                                                                                          \
+        if (wide) k_conv<KS, CI, CO, EPI, 4><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w);    \
+        else if (narrow) k_conv<KS, CI, CO, EPI, 1, 1><<<gn, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w); \
+        else k_conv<KS, CI, CO, EPI, 1><<<g, 256, 0, s>>>(in, Wf[k], bias, outp, n, h, w);         \
     } while (0)
         if (epi == EPI_F32) {
             if (l.ks == 1 && l.cin == 32 && l.cout == 16) SUR_CONV(1, 32, 16, EPI_F32);
@@ -699,12 +710,14 @@ struct Net {
         const LayerSpec& l = kSpec[k];
         const long n = npix(l.level, B);
         const int h = H >> l.level, w = W >> l.level;
-        const bool wide = conv_wide(n);
+        const bool wide = conv_wide(n), narrow = conv_narrow(n);
         const unsigned g = grid1(n, wide ? 256 : 64);
-#define SUR_DG(KS, CI, CO)                                                                              \
-    do {                                                                                                \
-        if (wide) k_conv<KS, CI, CO, EPI_LINEAR, 4><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w); \
-        else k_conv<KS, CI, CO, EPI_LINEAR, 1><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w);      \
+        const unsigned gn = grid1(n * (l.cin / 16), 64);  // the data gradient's outputs: cin channels
+#define SUR_DG(KS, CI, CO)                                                                                       \
+    do {                                                                                                         \
+        if (wide) k_conv<KS, CI, CO, EPI_LINEAR, 4><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w);          \
+        else if (narrow) k_conv<KS, CI, CO, EPI_LINEAR, 1, 1><<<gn, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w); \
+        else k_conv<KS, CI, CO, EPI_LINEAR, 1><<<g, 256, 0, s>>>(gz, Wb[k], nullptr, gx, n, h, w);               \
     } while (0)
         if (l.ks == 1 && l.cout == 16 && l.cin == 32) SUR_DG(1, 16, 32);
         else if (l.ks == 3 && l.cout == 32 && l.cin == 32) SUR_DG(3, 32, 32);
