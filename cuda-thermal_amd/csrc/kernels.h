@@ -96,6 +96,10 @@ struct State {
     // over the run the steps that took it (w > 0) and the sum of their w.
     double guide_w, guide_w_sum;
     int guide_steps;
+    // Learned steady start judged on the device: both starting relative
+    // residuals and whether the lifted guess was kept.
+    double lift_rp, lift_rl;
+    int lift_used;
 };
 
 // Block coordinates of a kernel body: the hardware ones (hw_blk), or, in a

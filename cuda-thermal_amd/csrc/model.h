@@ -87,6 +87,12 @@ int read_tmap(const std::string& path, std::vector<double>& vals, int& rows, int
 int read_tflp(const std::string& path, std::vector<std::string>& names, std::vector<double>& vals);
 Case builtin_example();
 int case_model(const Case& c, int nx, int ny, bool with_tsv, Model& m);
+// The surrogate's forward pass on a device input, queued on `stream` (a
+// hipStream_t; surrogate.hip): *out receives its output field [H][W][C] in
+// device memory, valid in that stream's order. The surrogate must live on
+// `device`.
+int surrogate_forward_on(th_surrogate* sur, const float* d_in, void* stream, int device, const float** out, int* H,
+                         int* W, int* C);
 
 }  // namespace th
 

@@ -575,9 +575,10 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
 }
 
 // Start of a solve: tolerance, iteration cap, cleared flags.
+// xz < 0: keep the device's flag (set by k_lift_decide).
 static __global__ void k_begin(State* st, double tol, int maxit, int xz) {
     st->xlag = 0;
-    st->xz = xz;
+    if (xz >= 0) st->xz = xz;
     st->tol = tol;
     st->maxit = maxit;
     st->iters = 0;
@@ -874,41 +875,77 @@ struct LiftMap {
     float w[64];
 };
 
-// x(i, j, l) = scale * the coarse field at the centre of cell (i, j),
-// bilinear over the Hc x Wc coarse cells (clamped at the edges), then
-// linear in z through the map. One thread per column (a slot of a row, both
-// colours): the bilinear weights are formed once per column and a wave
-// stores each plane as one run of a half-row; the same arithmetic per cell
-// as one thread per cell, so the same values. 1024^2 x 16 from 256^2: 99 ->
-// 86 us. (Issuing all planes' loads before the first store took it to 344
-// us: the gathers of a wave touch ~32 coarse pixels each, and 128 of them in
-// flight per lane thrash the L1.)
-static __global__ __launch_bounds__(256) void k_lift(Geo g, const float* __restrict__ f, int Hc, int Wc, int C,
-                                                     LiftMap zm, double scale, double* __restrict__ x) {
-    const int q = blockIdx.x * kColsX + threadIdx.x;
+// The coarse field taken to the solver's planes at every coarse cell, in
+// FP64: fz[l][y][x] = (1 - w[l]) f[y][x][lo[l]] + w[l] f[y][x][lo[l] + 1]
+// (f[y][x][lo[l]] alone where w[l] = 0).
+static __global__ __launch_bounds__(256) void k_lift_z(const float* __restrict__ f, int Hc, int Wc, int C, int nz,
+                                                       LiftMap zm, double* __restrict__ fz) {
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    const long hw = (long)Hc * Wc;
+    if (k >= hw * nz) return;
+    const int l = (int)(k / hw);
+    const long pix = k - l * hw;
+    const float* p = f + pix * C + zm.lo[l];
+    const double w = zm.w[l];
+    fz[k] = w > 0.0 ? (1.0 - w) * p[0] + w * p[1] : (double)p[0];
+}
+
+// x(i, j, l) = scale * fz[l] at the centre of cell (i, j), bilinear over the
+// Hc x Wc coarse cells (clamped at the edges). One thread per column of
+// `colour` (-1: both colours): the bilinear weights are formed once per
+// column, and a wave stores each plane as one run of a half-row and reads
+// the two coarse rows it needs as short runs of fz. Before a line
+// relaxation only the black columns are lifted: the red relaxation solves
+// every red column from its black neighbours alone, so lifted red values
+// would never be read. 1024^2 x 16 from 256^2: 83 us for both colours with
+// 8 gathers per plane and cell from the channel-interleaved field, against
+// 23 us for the black columns from fz (plus 7 us for k_lift_z); the same
+// values.
+static __global__ __launch_bounds__(256) void k_lift(Geo g, const double* __restrict__ fz, int Hc, int Wc,
+                                                     double scale, int colour, double* __restrict__ x) {
+    const int q0 = blockIdx.x * kColsX + threadIdx.x;
     const int j = blockIdx.y * kColsY + threadIdx.y;
+    const int q = colour < 0 ? q0 : colour * g.hx + q0;
     int i;
-    if (j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
+    if (j >= g.ny || q0 >= (colour < 0 ? 2 * g.hx : g.hx) || !slot_column(g, q, j, i)) return;
     const double xc = fmin(fmax((i + 0.5) * Wc / g.nx - 0.5, 0.0), (double)(Wc - 1));
     const double yc = fmin(fmax((j + 0.5) * Hc / g.ny - 0.5, 0.0), (double)(Hc - 1));
     const int x0 = (int)xc, y0 = (int)yc;
     const int x1 = x0 + 1 < Wc ? x0 + 1 : x0, y1 = y0 + 1 < Hc ? y0 + 1 : y0;
     const double fx = xc - x0, fy = yc - y0;
-    const float* r00 = f + ((long)y0 * Wc + x0) * C;
-    const float* r01 = f + ((long)y0 * Wc + x1) * C;
-    const float* r10 = f + ((long)y1 * Wc + x0) * C;
-    const float* r11 = f + ((long)y1 * Wc + x1) * C;
+    const long hw = (long)Hc * Wc;
+    const double* r00 = fz + (long)y0 * Wc + x0;
+    const double* r01 = fz + (long)y0 * Wc + x1;
+    const double* r10 = fz + (long)y1 * Wc + x0;
+    const double* r11 = fz + (long)y1 * Wc + x1;
     const long c0 = g.pos(i, j);
     for (int l = 0; l < g.nz; ++l) {
-        const int lo = zm.lo[l];
-        const double w = zm.w[l];
-        auto at = [&](const float* r) {
-            const float* p = r + lo;
-            return w > 0.0 ? (1.0 - w) * p[0] + w * p[1] : (double)p[0];
-        };
-        const double v = (1.0 - fy) * ((1.0 - fx) * at(r00) + fx * at(r01)) + fy * ((1.0 - fx) * at(r10) + fx * at(r11));
+        const long o = l * hw;
+        const double v = (1.0 - fy) * ((1.0 - fx) * r00[o] + fx * r01[o]) + fy * ((1.0 - fx) * r10[o] + fx * r11[o]);
         x[(long)l * g.plane + c0] = scale * v;
     }
+}
+
+// The learned start judged on the device (the fallback of set_x_lifted
+// without a host round trip): the plain start's relative residual (1 for
+// x = 0) and the lifted guess's, each from the FIN_CHECK totals of the
+// residual kernel just run; the guess is kept if its residual is the
+// smaller, and the next solve's start then reads x (xz = 0).
+static __global__ void k_lift_plain(State* st, int x_zero) {
+    st->lift_rp = x_zero ? 1.0 : sqrt(st->chk_rr / st->chk_bb);
+}
+static __global__ void k_lift_decide(State* st, int x_zero) {
+    const double rl = sqrt(st->chk_rr / st->chk_bb);
+    const int keep = rl < st->lift_rp;
+    st->lift_rl = rl;
+    st->lift_used = keep;
+    st->xz = keep ? 0 : x_zero;
+}
+// x = v where the guess was kept.
+static __global__ void k_copy_if_kept(long n, const double* __restrict__ v, double* __restrict__ x,
+                                      const State* __restrict__ st) {
+    if (!st->lift_used) return;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) x[k] = v[k];
 }
 
 // One colour of a column line relaxation of the steady operator in FP64:

@@ -114,6 +114,8 @@ struct SolverBase {
     virtual int set_guide_lifted(const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale,
                                  int passes) = 0;
     virtual int guide_stats(int* steps, double* w_mean) = 0;
+    virtual int set_x_learned(th_surrogate* sur, const float* d_in, const LiftMap& zm, double scale, int passes) = 0;
+    virtual int lifted_stats(int* used, double* rp, double* rl) = 0;
 };
 
 template <class T>
@@ -383,14 +385,17 @@ struct Solver final : SolverBase {
         }
         return TH_OK;
     }
-    bool x_zero = true;  // x is known to be 0 (set_x_zero, nothing since)
+    bool x_zero = true;      // x is known to be 0 (set_x_zero, nothing since)
+    bool xz_device = false;  // the device decided whether x is still 0 (set_x_learned)
     int set_x_zero() override {
         HIPTRY(hipMemsetAsync(x, 0, n0 * sizeof(double), s));
         x_zero = true;
+        xz_device = false;
         return TH_OK;
     }
     int set_x(const double* host) override {
         x_zero = false;
+        xz_device = false;
         return upload_split<double, double>(g0, g0.nz, host, x);
     }
     int set_rhs_steady() override {
@@ -814,8 +819,9 @@ struct Solver final : SolverBase {
 
     int solve(double tol, int maxit, th_stats* out) override {
         int rc;
-        const int xz = x_zero ? 1 : 0;  // the start residual is b
+        const int xz = xz_device ? -1 : x_zero ? 1 : 0;  // 1: the start residual is b
         x_zero = false;
+        xz_device = false;
         k_begin<<<1, 1, 0, s>>>(st, tol, maxit, xz);
         HIPTRY(hipGetLastError());
         if ((rc = run_start()) || (rc = fetch_state())) return rc;
@@ -994,6 +1000,7 @@ struct Solver final : SolverBase {
         if (keep) {
             HIPTRY(hipMemcpyAsync(x, v, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
             x_zero = false;
+            xz_device = false;
         }
         HIPTRY(hipStreamSynchronize(s));
         if (used) *used = keep;
@@ -1011,10 +1018,59 @@ struct Solver final : SolverBase {
         HIPTRY(hipStreamSynchronize(s));
         return TH_OK;
     }
-    // v = the coarse field lifted to the fine grid, then `passes` red-black
-    // column line relaxations of the steady operator.
+    // The learned steady start in one asynchronous call: the surrogate's
+    // forward pass on this solver's stream, the lift and set_x_lifted's
+    // fallback judged on the device (k_lift_plain, k_lift_decide,
+    // k_copy_if_kept), so nothing waits for the host before the solve; the
+    // judgement is read afterwards (lifted_stats).
+    int set_x_learned(th_surrogate* sur, const float* d_in, const LiftMap& zm, double scale, int passes) override {
+        int rc;
+        const float* field = nullptr;
+        int Hc = 0, Wc = 0, C = 0;
+        if ((rc = surrogate_forward_on(sur, d_in, (void*)s, opt.device, &field, &Hc, &Wc, &C))) return rc;
+        const int xz0 = x_zero ? 1 : 0;
+        if (!x_zero && ((rc = launch_residual<RES_CHECK>(P, x, part)) || (rc = finalize<3, FIN_CHECK>(nblk_res))))
+            return rc;
+        k_lift_plain<<<1, 1, 0, s>>>(st, xz0);
+        HIPTRY(hipGetLastError());
+        double* v = Ap;  // free between solves
+        if ((rc = lift_into(v, field, Hc, Wc, C, zm, scale, passes))) return rc;
+        if ((rc = launch_residual<RES_CHECK>(P, v, part)) || (rc = finalize<3, FIN_CHECK>(nblk_res))) return rc;
+        k_lift_decide<<<1, 1, 0, s>>>(st, xz0);
+        HIPTRY(hipGetLastError());
+        k_copy_if_kept<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, v, x, st);
+        HIPTRY(hipGetLastError());
+        x_zero = false;
+        xz_device = true;
+        return TH_OK;
+    }
+    int lifted_stats(int* used, double* rp, double* rl) override {
+        int rc;
+        if ((rc = fetch_state())) return rc;
+        if (used) *used = h_st->lift_used;
+        if (rp) *rp = h_st->lift_rp;
+        if (rl) *rl = h_st->lift_rl;
+        return TH_OK;
+    }
+    // v = the coarse field lifted to the fine grid (its planes first taken to
+    // the solver's planes on the coarse grid, k_lift_z, then bilinear over the
+    // die, k_lift), then `passes` red-black column line relaxations of the
+    // steady operator: with passes > 0 only the black columns are lifted,
+    // since the first (red) relaxation overwrites every red column.
+    double* lift_fz = nullptr;
+    size_t lift_fz_n = 0;
     int lift_into(double* v, const float* field, int Hc, int Wc, int C, const LiftMap& zm, double scale, int passes) {
-        k_lift<<<col_grid(2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, field, Hc, Wc, C, zm, scale, v);
+        int rc;
+        const size_t nfz = (size_t)Hc * Wc * g0.nz;
+        if (nfz > lift_fz_n) {
+            if ((rc = dalloc(&lift_fz, nfz))) return rc;
+            lift_fz_n = nfz;
+        }
+        k_lift_z<<<(unsigned)((nfz + 255) / 256), 256, 0, s>>>(field, Hc, Wc, C, g0.nz, zm, lift_fz);
+        HIPTRY(hipGetLastError());
+        const int colour = passes > 0 ? 1 : -1;
+        k_lift<<<col_grid(passes > 0 ? g0.hx : 2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, lift_fz, Hc, Wc, scale, colour,
+                                                                                      v);
         HIPTRY(hipGetLastError());
         const Ops<T> o = L[0].ops();
         for (int k = 0; k < passes; ++k)

@@ -303,6 +303,18 @@ int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int
     return s->impl->set_x_lifted(field, Hc, Wc, C, zm, scale, passes, used, relres_plain, relres_lift);
 }
 
+int th_solver_set_x_learned(th_solver* s, th_surrogate* sur, const float* d_in, int nzc, const double* zc_coarse,
+                            const double* zc_fine, double scale, int passes) {
+    TH_S(s);
+    if (!sur || !d_in || !zc_coarse || !zc_fine) return fail(TH_ERR_ARG, "null argument");
+    if (nzc < 1 || nzc > 16 || passes < 0) return fail(TH_ERR_ARG, "bad coarse field shape");
+    return s->impl->set_x_learned(sur, d_in, lift_map(s->impl->nz(), nzc, zc_coarse, zc_fine), scale, passes);
+}
+int th_solver_lifted_stats(th_solver* s, int* used, double* relres_plain, double* relres_lift) {
+    TH_S(s);
+    return s->impl->lifted_stats(used, relres_plain, relres_lift);
+}
+
 // transient3d.h:48-90, stepped on the device (Solver::run_transient).
 int th_run_transient(th_solver* s, const th_transient_opts* o, th_step_record* recs, int rec_cap, int* nsteps,
                      double* T_final, int* all_converged, double* solve_ms) {

@@ -530,6 +530,7 @@ struct SplitMix {
 
 struct Net {
     int H = 0, W = 0, Bcap = 0;
+    int dev = 0;  // the device it was created on
     hipStream_t s = nullptr;
     long nparam = 0;
     long off[kLayers];
@@ -585,6 +586,7 @@ struct Net {
     int init(int h, int w, uint64_t seed) {
         H = h;
         W = w;
+        SURTRY(hipGetDevice(&dev));
         SURTRY(hipStreamCreate(&s));
         nparam = 0;
         for (int k = 0; k < kLayers; ++k) {
@@ -976,6 +978,49 @@ int th_surrogate_predict_device_input(th_surrogate* s, const float* d_in, const 
     *d_out = n.out;
     return TH_OK;
 }
+
+}  // extern "C"
+
+namespace th {
+// The forward pass of th_surrogate_predict_device_input queued on another
+// stream: that stream first waits for the surrogate's own queue, the
+// network's kernels run on it, and the surrogate's queue then waits for
+// them, so the two stay ordered without the host.
+int surrogate_forward_on(th_surrogate* sur, const float* d_in, void* stream, int device, const float** out, int* H,
+                         int* W, int* C) {
+    if (!sur || !d_in || !stream || !out) return fail(TH_ERR_ARG, "bad argument");
+    sur::Net& n = sur->net;
+    if (n.dev != device) return fail(TH_ERR_ARG, "the surrogate and the solver are on different devices");
+    int rc;
+    if ((rc = n.ensure_batch(1))) return rc;
+    hipStream_t other = (hipStream_t)stream;
+    hipEvent_t ev = nullptr;
+    SURTRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, n.s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(other, ev, 0);
+    hipStream_t own = n.s;
+    if (e == hipSuccess) {
+        n.s = other;
+        const long npx = n.npix(0, 1) * 16;
+        sur::k_pack_input<<<sur::grid1(npx), 256, 0, n.s>>>(d_in, n.x0, npx);
+        e = hipGetLastError();
+        if (e == hipSuccess) rc = n.forward(1);
+        n.s = own;
+    }
+    if (e == hipSuccess && !rc) e = hipEventRecord(ev, other);
+    if (e == hipSuccess && !rc) e = hipStreamWaitEvent(own, ev, 0);
+    (void)hipEventDestroy(ev);  // released once the waits are resolved
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(TH_ERR_HIP, std::string("surrogate on the solver's stream: ") + hipGetErrorString(e));
+    *out = n.out;
+    if (H) *H = n.H;
+    if (W) *W = n.W;
+    if (C) *C = 16;
+    return TH_OK;
+}
+}  // namespace th
+
+extern "C" {
 
 int th_surrogate_grad(th_surrogate* s, const float* in, const float* tgt, int batch, double* loss, float* grad) {
     TH_SUR(s);

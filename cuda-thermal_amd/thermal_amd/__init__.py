@@ -175,6 +175,8 @@ SIGNATURES = {
     "th_surrogate_time_forward": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "th_solver_set_x_lifted": (C.c_int, [_vp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double,
                                          C.c_int, _ip, _dp, _dp]),
+    "th_solver_set_x_learned": (C.c_int, [_vp, _vp, C.c_void_p, C.c_int, _dp, _dp, C.c_double, C.c_int]),
+    "th_solver_lifted_stats": (C.c_int, [_vp, _ip, _dp, _dp]),
 }
 
 _lib = None
@@ -592,6 +594,22 @@ class Solver:
         used, rp, rl = C.c_int(), C.c_double(), C.c_double()
         _check(lib().th_solver_set_x_lifted(self.h, C.cast(d_field, _fp), Hc, Wc, C_, len(zc), _d(zc), _d(zf),
                                             scale, passes, C.byref(used), C.byref(rp), C.byref(rl)))
+        return bool(used.value), rp.value, rl.value
+
+    def set_x_learned(self, sur, d_in, zc_coarse, zc_fine, scale=1.0, passes=1):
+        """The learned start in one asynchronous call (th_solver_set_x_learned):
+        the surrogate `sur` predicts from its device input d_in on this
+        solver's stream, and the lifted guess is kept or dropped on the
+        device; lifted_stats() reads the judgement afterwards."""
+        zc = np.ascontiguousarray(zc_coarse, np.float64)
+        zf = np.ascontiguousarray(zc_fine, np.float64)
+        _check(lib().th_solver_set_x_learned(self.h, sur.h, C.c_void_p(d_in), len(zc), _d(zc), _d(zf), scale,
+                                             passes))
+
+    def lifted_stats(self):
+        """(used, relres_plain, relres_lift) of the last set_x_learned."""
+        used, rp, rl = C.c_int(), C.c_double(), C.c_double()
+        _check(lib().th_solver_lifted_stats(self.h, C.byref(used), C.byref(rp), C.byref(rl)))
         return bool(used.value), rp.value, rl.value
 
 

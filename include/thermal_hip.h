@@ -383,6 +383,19 @@ TH_API int th_surrogate_time_forward(th_surrogate* s, int batch, int reps, doubl
 TH_API int th_solver_set_x_lifted(th_solver* s, const float* field, int Hc, int Wc, int C, int nzc,
                                   const double* zc_coarse, const double* zc_fine, double scale, int passes,
                                   int* used, double* relres_plain, double* relres_lift);
+/* The learned steady start in one asynchronous call (no counterpart in the
+ * reference): the surrogate's forward pass on d_in (its input in device
+ * memory, as th_surrogate_predict_device_input takes it), the lift of its
+ * output and th_solver_set_x_lifted's fallback, all queued on the solver's
+ * stream and judged on the device, so the host waits for nothing before the
+ * next th_solver_solve. The surrogate must have been created on the
+ * solver's device; its output channels 0..nzc-1 (nzc <= 16) are the planes
+ * with centres zc_coarse. th_solver_lifted_stats then reads the judgement
+ * (waiting for the device): *used = 1 if the lifted guess was kept, and
+ * both starting relative residuals. */
+TH_API int th_solver_set_x_learned(th_solver* s, th_surrogate* sur, const float* d_in, int nzc,
+                                   const double* zc_coarse, const double* zc_fine, double scale, int passes);
+TH_API int th_solver_lifted_stats(th_solver* s, int* used, double* relres_plain, double* relres_lift);
 
 #ifdef __cplusplus
 }

@@ -258,3 +258,97 @@ def test_predict_device_input_matches_host_input():
     b = fetch(sur.predict_device_input(d.ptr))
     assert np.array_equal(a, b)
     sur.close()
+
+
+def _lift_np(f, nx, ny, nz, zc, zf, scale):
+    """The lift restated in numpy (float64, the kernels' operation order):
+    each fine plane from the coarse channels through the plane map (linear
+    between the two coarse planes around it, the nearest outside them, the
+    weight rounded to float32), then bilinear over the die from cell
+    centres, clamped at the edges. Returns [nz][ny][nx]."""
+    Hc, Wc, _ = f.shape
+    lo, w = np.zeros(nz, int), np.zeros(nz, np.float32)
+    for l in range(nz):
+        z, k = zf[l], 0
+        while k + 1 < len(zc) and zc[k + 1] <= z:
+            k += 1
+        if k + 1 >= len(zc) or z <= zc[0]:
+            lo[l] = 0 if z <= zc[0] else len(zc) - 1
+        else:
+            lo[l], w[l] = k, np.float32((z - zc[k]) / (zc[k + 1] - zc[k]))
+    f64 = f.astype(np.float64)
+    fz = np.empty((nz, Hc, Wc))
+    for l in range(nz):
+        wl = np.float64(w[l])
+        fz[l] = (1.0 - wl) * f64[..., lo[l]] + wl * f64[..., lo[l] + 1] if wl > 0.0 else f64[..., lo[l]]
+    xc = np.clip((np.arange(nx) + 0.5) * Wc / nx - 0.5, 0.0, Wc - 1.0)
+    yc = np.clip((np.arange(ny) + 0.5) * Hc / ny - 0.5, 0.0, Hc - 1.0)
+    x0, y0 = xc.astype(int), yc.astype(int)
+    x1, y1 = np.minimum(x0 + 1, Wc - 1), np.minimum(y0 + 1, Hc - 1)
+    fx, fy = (xc - x0)[None, :], (yc - y0)[:, None]
+    out = np.empty((nz, ny, nx))
+    for l in range(nz):
+        F = fz[l]
+        v = (1.0 - fy) * ((1.0 - fx) * F[y0][:, x0] + fx * F[y0][:, x1]) + fy * ((1.0 - fx) * F[y1][:, x0] + fx * F[y1][:, x1])
+        out[l] = scale * v
+    return out
+
+
+def test_lift_matches_numpy_restatement():
+    """The lifted field itself (no relaxation) against its numpy restatement,
+    bit for bit: a 20 x 24 coarse field of 7 planes whose centres lie inside
+    the stack (so the bottom and top planes take the nearest one and the
+    others interpolate) onto 64 x 64 x 10, scaled."""
+    n = 64
+    m = W.build(TS.sample_spec(9, 7003), n)
+    zf = np.asarray(TS.plane_centres(W.config0_layers()), np.float64)
+    zc = np.linspace(zf[1] * 0.9 + zf[0] * 0.1, zf[-2] * 0.9 + zf[-1] * 0.1, 7)
+    rng = np.random.default_rng(5)
+    f = (10.0 + 5.0 * rng.random((20, 24, 16))).astype(np.float32)
+    d = DeviceArray(f)
+    s = T.Solver(m, precision="mixed")
+    s.set_x(np.full(m.n, 1e9))  # so that the lifted field is kept
+    used, _, _ = s.set_x_lifted(d.ptr, 20, 24, 16, zc, zf, scale=1.7, passes=0)
+    assert used
+    got = s.get_x().reshape(m.nz, n, n)
+    ref = _lift_np(f, n, n, m.nz, zc, zf, 1.7)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    s.close()
+
+
+@pytest.mark.parametrize("bad_start", [False, True])
+def test_learned_start_async_matches_sync(bad_start):
+    """th_solver_set_x_learned (forward pass, lift and fallback queued on the
+    solver's stream, judged on the device) against the synchronous
+    predict_device_input + set_x_lifted on the same input: the same judgement
+    and starting residuals, then the same iterations and field bit for bit;
+    once with the plain start x = 0 (the lifted guess of an untrained
+    network loses) and once with a plain start so bad that it is kept."""
+    n = 64
+    m = W.build(TS.sample_spec(9, 7004), n)
+    zf = TS.plane_centres(W.config0_layers())
+    sur = T.Surrogate(n, n, seed=11)
+    feats = DeviceArray(features_batch(1, n, 8)[0])
+    out = []
+    for mode in ("sync", "async"):
+        s = T.Solver(m, precision="mixed", true_residual=True)
+        if bad_start:
+            s.set_x(np.full(m.n, 1e9))
+        else:
+            s.set_x_zero()
+        if mode == "sync":
+            d = sur.predict_device_input(feats.ptr)
+            stats = s.set_x_lifted(d, n, n, 16, TS.COARSE_Z, zf, scale=T.Surrogate.T_SCALE, passes=1)
+        else:
+            s.set_x_learned(sur, feats.ptr, TS.COARSE_Z, zf, scale=T.Surrogate.T_SCALE, passes=1)
+        s.set_rhs_steady()
+        st = s.solve(1e-10, 200)
+        if mode == "async":
+            stats = s.lifted_stats()
+        out.append((stats, st["iters"], st["converged"], s.get_x()))
+        s.close()
+    (sa, ia, ca, xa), (sb, ib, cb, xb) = out
+    assert sa == sb and sa[0] == bad_start, (sa, sb)
+    assert ia == ib and ca and cb
+    assert np.array_equal(xa, xb)
+    sur.close()
