@@ -12,7 +12,10 @@
 
 namespace th {
 
-enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2 };
+// RES_TRUE: RES_MG's residual and partials without its writes (the
+// true-residual test at the end of a solve, whose r and b0 nothing reads: a
+// restart recomputes them in its start graph).
+enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2, RES_TRUE = 3 };
 enum { UP_MG = 0, UP_MG_FLEX = 1, UP_JACOBI = 2 };
 enum {
     FIN_RES = 0,
@@ -572,6 +575,17 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
         st->e_out = tot[1];
         st->e_tmax = tot[2];
     }
+}
+
+// The convergence test of an iteration's x, r update on its own, for the
+// last iteration of a batch the host queues ahead (its V-cycle, dot and p
+// update wait for the host's look at the state, so that a converged solve
+// leaves no queue of early-returning launches behind): consume_convergence
+// as the head of the V-cycle runs it, with its thread count (256), so the
+// sums and the state are the same.
+static __global__ __launch_bounds__(256) void k_consume_iter(const double* __restrict__ part, int nblk, State* st) {
+    if (st->done) return;
+    consume_convergence(part, nblk, CONS_ITER, st, true);
 }
 
 // Start of a solve: tolerance, iteration cap, cleared flags.

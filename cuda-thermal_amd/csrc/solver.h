@@ -135,12 +135,20 @@ struct Solver final : SolverBase {
     State* h_st = nullptr;  // pinned mirror
     std::vector<DevLevel<T>> L;
     hipGraphExec_t g_start = nullptr, g_iter = nullptr;
+    // An iteration in two parts (split_iter: multigrid with a V-cycle whose
+    // head is a 256-thread sweep): apply, update and the convergence test;
+    // then V-cycle, dot and p update. mid_iter: the first part of the last
+    // queued iteration has run, the second has not.
+    hipGraphExec_t g_iter_a = nullptr, g_iter_b = nullptr;
+    bool split_iter = false, mid_iter = false;
     std::vector<void*> owned;
 
     ~Solver() override {
         if (s) (void)hipStreamSynchronize(s);
         if (g_start) (void)hipGraphExecDestroy(g_start);
         if (g_iter) (void)hipGraphExecDestroy(g_iter);
+        if (g_iter_a) (void)hipGraphExecDestroy(g_iter_a);
+        if (g_iter_b) (void)hipGraphExecDestroy(g_iter_b);
         for (void* ptr : owned) (void)hipFree(ptr);
         if (h_st) (void)hipHostFree(h_st);
         if (s) (void)hipStreamDestroy(s);
@@ -323,9 +331,13 @@ struct Solver final : SolverBase {
         if ((rc = set_dt(0.0))) return rc;
         HIPTRY(hipStreamSynchronize(s));
         lap(setup.factor_ms);
+        split_iter = mg() && tail_k0 > 0;
         if (opt.use_graphs && mg()) {
             if ((rc = capture(&g_start, true))) return rc;
             if ((rc = capture(&g_iter, false))) return rc;
+            if (split_iter && ((rc = capture_fn(&g_iter_a, [&]() { return enqueue_iter_a(); })) ||
+                               (rc = capture_fn(&g_iter_b, [&]() { return enqueue_iter_b(); }))))
+                return rc;
         }
         lap(setup.graphs_ms);
         setup.total_ms = std::chrono::duration<double, std::milli>(clk::now() - t_start).count();
@@ -708,6 +720,44 @@ struct Solver final : SolverBase {
         return finalize<3, FIN_RES_JACOBI>(nblk_stencil, part_res);
     }
 
+    // Multigrid iteration, first part: Ap; x, r update (alpha formed from the
+    // apply partials).
+    int enqueue_apply_update() {
+        const unsigned vb = vec_blocks(n0);
+        int rc;
+        if ((rc = launch_apply_kernel(pm))) return rc;
+        if (opt.beta == TH_BETA_FLEXIBLE)
+            k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x, nullptr, nullptr,
+                                                               st, part_upd, part_apply, (int)nblk_apply);
+        else
+            k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr, nullptr, st,
+                                                          part_upd, part_apply, (int)nblk_apply);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+    // Second part: z = M^-1 r (V-cycle, its head consuming the update's
+    // partials: the convergence test, unless k_consume_iter ran it); r.z;
+    // p = z + beta p (beta formed from the dot partials) with the lagged
+    // x += alpha p.
+    int enqueue_precondition(Cons cs) {
+        const unsigned vb = vec_blocks(n0);
+        int rc;
+        if ((rc = vcycle(0, cs))) return rc;
+        k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
+        HIPTRY(hipGetLastError());
+        k_pupdate<T, 0, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, pm, x, st, part_dot, (int)vb);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int enqueue_iter_a() {
+        int rc;
+        if ((rc = enqueue_apply_update())) return rc;
+        k_consume_iter<<<1, 256, 0, s>>>(part_upd, (int)vec_blocks(n0), st);
+        HIPTRY(hipGetLastError());
+        return TH_OK;
+    }
+    int enqueue_iter_b() { return enqueue_precondition(Cons{nullptr, 0, CONS_NONE}); }
+
     // One PCG iteration: Ap; x, r update (alpha formed from the apply
     // partials); convergence test + z = M^-1 r (V-cycle); r.z; p = z + beta p
     // (beta formed from the dot partials). Multigrid: 3 + V-cycle launches.
@@ -715,21 +765,8 @@ struct Solver final : SolverBase {
         const unsigned vb = vec_blocks(n0);
         int rc;
         if (mg()) {
-            if ((rc = launch_apply_kernel(pm))) return rc;
-            if (opt.beta == TH_BETA_FLEXIBLE)
-                k_update<T, UP_MG_FLEX><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, L[0].x, nullptr,
-                                                                   nullptr, st, part_upd, part_apply,
-                                                                   (int)nblk_apply);
-            else
-                k_update<T, UP_MG><<<vb, kVecThreads, 0, s>>>(n0, nullptr, Ap, x, r, L[0].b, nullptr, nullptr, nullptr,
-                                                              st, part_upd, part_apply, (int)nblk_apply);
-            HIPTRY(hipGetLastError());
-            if ((rc = vcycle(0, Cons{part_upd, (int)vb, CONS_ITER}))) return rc;
-            k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
-            HIPTRY(hipGetLastError());
-            k_pupdate<T, 0, T><<<vb, kVecThreads, 0, s>>>(n0, L[0].x, pm, x, st, part_dot, (int)vb);
-            HIPTRY(hipGetLastError());
-            return TH_OK;
+            if ((rc = enqueue_apply_update())) return rc;
+            return enqueue_precondition(Cons{part_upd, (int)vb, CONS_ITER});
         }
         if ((rc = launch_apply())) return rc;
         k_update<T, UP_JACOBI><<<vb, kVecThreads, 0, s>>>(n0, p, Ap, x, r, nullptr, nullptr, invd, z, st, part_upd,
@@ -748,6 +785,7 @@ struct Solver final : SolverBase {
     }
 
     int run_start() {
+        mid_iter = false;
         if (g_start) {
             HIPTRY(hipGraphLaunch(g_start, s));
             return TH_OK;
@@ -760,6 +798,27 @@ struct Solver final : SolverBase {
             return TH_OK;
         }
         return enqueue_iter();
+    }
+    // The last iteration of a queued batch: its first part only where the
+    // iteration splits (the host then looks at the state), else whole.
+    int run_iter_last() {
+        if (!split_iter) return run_iter();
+        mid_iter = true;
+        if (g_iter_a) {
+            HIPTRY(hipGraphLaunch(g_iter_a, s));
+            return TH_OK;
+        }
+        return enqueue_iter_a();
+    }
+    // The second part of an iteration run_iter_last left open.
+    int finish_iter() {
+        if (!mid_iter) return TH_OK;
+        mid_iter = false;
+        if (g_iter_b) {
+            HIPTRY(hipGraphLaunch(g_iter_b, s));
+            return TH_OK;
+        }
+        return enqueue_iter_b();
     }
 
     // Residual replacement: r := b - A x, the Krylov state (p, rz, beta)
@@ -789,7 +848,13 @@ struct Solver final : SolverBase {
         int rc;
         double rate = 0.0;  // relres reduction per iteration, once known
         replace = replace && opt.true_residual && mg() && h_st->tol < 1e-8;
+        bool ran = false;  // a batch of this call has run (the replacement waits for one)
         while (!h_st->done && h_st->iters < maxit) {
+            if ((rc = finish_iter())) return rc;  // x complete again
+            if (ran && replace && h_st->relres < kReplaceAt * h_st->tol) {
+                if ((rc = replace_residual())) return rc;
+                replace = false;
+            }
             int k = first;
             first = 1;
             if (rate > 0.0 && rate < 1.0 && h_st->relres > 0.0) {
@@ -803,17 +868,16 @@ struct Solver final : SolverBase {
             k = std::min(k, maxit - h_st->iters);
             const int it0 = h_st->iters;
             const double rel0 = h_st->relres;
-            for (int q = 0; q < k; ++q)
+            for (int q = 0; q < k - 1; ++q)
                 if ((rc = run_iter())) return rc;
+            if ((rc = run_iter_last())) return rc;
             if ((rc = fetch_state())) return rc;
             const int did = h_st->iters - it0;
             if (did > 0 && rel0 > 0.0 && h_st->relres > 0.0) rate = std::pow(h_st->relres / rel0, 1.0 / did);
             if (did == 0) break;  // nothing ran: the device is done
-            if (replace && !h_st->done && h_st->relres < kReplaceAt * h_st->tol) {
-                if ((rc = replace_residual())) return rc;
-                replace = false;
-            }
+            ran = true;
         }
+        mid_iter = false;  // done (k_finish applies the lagged x) or out of iterations
         return TH_OK;
     }
 
@@ -850,8 +914,11 @@ struct Solver final : SolverBase {
             HIPTRY(hipGetLastError());
             // The true residual of x and the convergence test on it, in one
             // kernel and a one-block finalize: when it is below tol the start
-            // graph (a V-cycle's worth of launches) is not replayed.
-            if ((rc = launch_residual<RES_MG>(b, x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
+            // graph (a V-cycle's worth of launches) is not replayed. The
+            // kernel writes nothing (RES_TRUE): the start graph of a restart
+            // computes r and the V-cycle's b itself. (With RES_MG's writes
+            // the test took 135 us at 1024^2 x 16, without them 83.)
+            if ((rc = launch_residual<RES_TRUE>(b, x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
                 (rc = fetch_state()))
                 return rc;
             if (h_st->done) break;
@@ -958,8 +1025,10 @@ struct Solver final : SolverBase {
                 dt = std::min(dt, o.t_end - t);
                 if (gh) HIPTRY(hipGraphLaunch(gh, s));
                 else if ((rc = enqueue_step_head(o.tol, o.maxit, o.predictor))) return rc;
-                for (int q = 0; q < std::min(prev_iters, o.maxit); ++q)
+                const int nq = std::min(prev_iters, o.maxit);
+                for (int q = 0; q < nq - 1; ++q)
                     if ((rc = run_iter())) return rc;
+                if (nq > 0 && (rc = run_iter_last())) return rc;
                 if ((rc = fetch_state())) return rc;
                 if ((rc = take_record(steps - 1))) return rc;
                 if ((rc = converge(o.maxit, 1))) return rc;
