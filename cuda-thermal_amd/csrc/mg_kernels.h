@@ -311,8 +311,14 @@ __global__ __launch_bounds__(256) void k_restrict(Geo f, Ops<T> o, const T* __re
     restrict_cell<T>(f, o, fb, fx, cg, cb, I, J, l, (T)st->inv_dt);
 }
 
-// Column factors (mg3d.h:46-61) for every column of a level: 1/pivot.
-template <class T>
+// Column factors (mg3d.h:46-61) for every column of a level: 1/pivot. NZ
+// is the plane-count bucket: planes past nz re-read the top plane, so every
+// plane's loads are unconditional and in flight together before the serial
+// recurrence (absent lateral faces point at the cell itself and add
+// nothing, as before). With the loads inside the recurrence, each plane
+// waited for its own: 30 us per launch on the 128^2 ... 16^2 levels at
+// nz = 16 whatever their size. The same arithmetic in the same order.
+template <class T, int NZ>
 __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__ finv, const State* __restrict__ st) {
     const int q = blockIdx.x * kColsX + threadIdx.x;
     const int j = blockIdx.y * kColsY + threadIdx.y;
@@ -320,23 +326,35 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
     if (!st->refactor || j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
     const Col c = make_col(g, i, j);
     const T inv_dt = (T)st->inv_dt;
-    T cp_prev = 0;
-    for (int l = 0; l < g.nz; ++l) {
-        const long base = (long)l * g.plane, id = base + c.c0;
+    const int nz = g.nz;
+    T dg[NZ], gu[NZ], gd[NZ];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        const int lc = l < nz ? l : nz - 1;
+        const long base = (long)lc * g.plane, id = base + c.c0;
+        const T ew = o.gE[base + c.pw], ee = o.gE[id], ns = o.gN[base + c.ps], nn = o.gN[id];
+        const T u = o.gU[id], dn = lc > 0 ? o.gU[id - g.plane] : o.gB[c.c0], cp = o.cap[id];
         T d = 0;
-        if (c.hw) d += o.gE[base + c.pw];
-        if (c.he) d += o.gE[id];
-        if (c.hs) d += o.gN[base + c.ps];
-        if (c.hn) d += o.gN[id];
-        d += o.gU[id];
-        d += (l > 0) ? o.gU[id - g.plane] : o.gB[c.c0];
-        d = d + o.cap[id] * inv_dt;
-        const T lower = (l > 0) ? -o.gU[id - g.plane] : T(0);
-        const T upper = (l < g.nz - 1) ? -o.gU[id] : T(0);
-        const T inv = T(1) / (d - lower * cp_prev);
-        finv[id] = inv;
-        cp_prev = upper * inv;
+        if (c.hw) d += ew;
+        if (c.he) d += ee;
+        if (c.hs) d += ns;
+        if (c.hn) d += nn;
+        d += u;
+        d += dn;
+        dg[l] = d + cp * inv_dt;
+        gu[l] = u;
+        gd[l] = dn;
     }
+    T cp_prev = 0;
+#pragma unroll
+    for (int l = 0; l < NZ; ++l)
+        if (l < nz) {
+            const T lower = (l > 0) ? -gd[l] : T(0);
+            const T upper = (l < nz - 1) ? -gu[l] : T(0);
+            const T inv = T(1) / (dg[l] - lower * cp_prev);
+            finv[(long)l * g.plane + c.c0] = inv;
+            cp_prev = upper * inv;
+        }
 }
 
 // ---- typed fine level ----------------------------------------------------------
@@ -347,8 +365,9 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
 // are built), and every kernel below does its dense counterpart's arithmetic
 // in the same order: results are identical.
 
-// 1/pivot of every type for the current dt (k_factor's arithmetic).
-template <class T>
+// 1/pivot of every type for the current dt (k_factor's arithmetic), every
+// plane's row read before the recurrence (NZ: the plane-count bucket).
+template <class T, int NZ>
 __global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const unsigned char* __restrict__ fl, int nt,
                                                      int nz, int nzp, const State* __restrict__ st) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,23 +377,33 @@ __global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const 
     const unsigned fb = fl[t];
     const bool hw = fb & 1, he = fb & 2, hs = fb & 4, hn = fb & 8;
     const T inv_dt = (T)st->inv_dt;
-    T cp_prev = 0;
-    for (int l = 0; l < nz; ++l) {
-        const T* r = rows + l * kTypeFields;
+    T dg[NZ], gu[NZ], gd[NZ];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        const int lc = l < nz ? l : nz - 1;
+        const T* r = rows + lc * kTypeFields;
+        const T dn = lc > 0 ? r[TF_U - kTypeFields] : gB;
         T d = 0;
         if (hw) d += r[TF_W];
         if (he) d += r[TF_E];
         if (hs) d += r[TF_S];
         if (hn) d += r[TF_N];
         d += r[TF_U];
-        d += (l > 0) ? r[TF_U - kTypeFields] : gB;
-        d = d + r[TF_C] * inv_dt;
-        const T lower = (l > 0) ? -r[TF_U - kTypeFields] : T(0);
-        const T upper = (l < nz - 1) ? -r[TF_U] : T(0);
-        const T inv = T(1) / (d - lower * cp_prev);
-        rows[l * kTypeFields + TF_FI] = inv;
-        cp_prev = upper * inv;
+        d += dn;
+        dg[l] = d + r[TF_C] * inv_dt;
+        gu[l] = r[TF_U];
+        gd[l] = dn;
     }
+    T cp_prev = 0;
+#pragma unroll
+    for (int l = 0; l < NZ; ++l)
+        if (l < nz) {
+            const T lower = (l > 0) ? -gd[l] : T(0);
+            const T upper = (l < nz - 1) ? -gu[l] : T(0);
+            const T inv = T(1) / (dg[l] - lower * cp_prev);
+            rows[l * kTypeFields + TF_FI] = inv;
+            cp_prev = upper * inv;
+        }
 }
 
 // sweep_column with the faces from the column's type. UNI: every column of

@@ -381,13 +381,36 @@ struct Solver final : SolverBase {
     // return at once unless the state asks for new factors.
     int enqueue_factors() {
         if (mg()) {
+            // a level with column types reads its factors from the type table
+            // only: its per-cell 1/pivot is never read
             for (DevLevel<T>& d : L) {
-                k_factor<T><<<col_grid(2 * d.g.hx, d.g.ny), col_block(), 0, s>>>(d.g, d.ops(), d.finv, st);
+                if (d.ntypes) continue;
+                const dim3 grid = col_grid(2 * d.g.hx, d.g.ny);
+#define TH_FACTOR(NZ) k_factor<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, st)
+                if (d.g.nz <= 4) TH_FACTOR(4);
+                else if (d.g.nz <= 8) TH_FACTOR(8);
+                else if (d.g.nz <= 12) TH_FACTOR(12);
+                else if (d.g.nz <= 16) TH_FACTOR(16);
+                else if (d.g.nz <= 24) TH_FACTOR(24);
+                else if (d.g.nz <= 32) TH_FACTOR(32);
+                else if (d.g.nz <= 48) TH_FACTOR(48);
+                else TH_FACTOR(64);
+#undef TH_FACTOR
                 HIPTRY(hipGetLastError());
             }
             for (DevLevel<T>& d : L)
                 if (d.ntypes) {
-                    k_factor_types<T><<<(d.ntypes + 63) / 64, 64, 0, s>>>(d.tab, d.tfl, d.ntypes, d.g.nz, d.g.nz, st);
+                    const unsigned nb = (unsigned)((d.ntypes + 63) / 64);
+#define TH_FTYPES(NZ) k_factor_types<T, NZ><<<nb, 64, 0, s>>>(d.tab, d.tfl, d.ntypes, d.g.nz, d.g.nz, st)
+                    if (d.g.nz <= 4) TH_FTYPES(4);
+                    else if (d.g.nz <= 8) TH_FTYPES(8);
+                    else if (d.g.nz <= 12) TH_FTYPES(12);
+                    else if (d.g.nz <= 16) TH_FTYPES(16);
+                    else if (d.g.nz <= 24) TH_FTYPES(24);
+                    else if (d.g.nz <= 32) TH_FTYPES(32);
+                    else if (d.g.nz <= 48) TH_FTYPES(48);
+                    else TH_FTYPES(64);
+#undef TH_FTYPES
                     HIPTRY(hipGetLastError());
                 }
         } else {
