@@ -15,7 +15,9 @@ namespace th {
 // RES_TRUE: RES_MG's residual and partials without its writes (the
 // true-residual test at the end of a solve, whose r and b0 nothing reads: a
 // restart recomputes them in its start graph).
-enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2, RES_TRUE = 3 };
+// RES_RHS: a backward-Euler step's start in one pass: b = P + C/dt x
+// (k_rhs_transient's arithmetic, stored to bw) and then RES_MG from it.
+enum { RES_MG = 0, RES_JACOBI = 1, RES_CHECK = 2, RES_TRUE = 3, RES_RHS = 4 };
 enum { UP_MG = 0, UP_MG_FLEX = 1, UP_JACOBI = 2 };
 enum {
     FIN_RES = 0,
@@ -39,7 +41,8 @@ __device__ __forceinline__ void residual_body(const Blk& B, const Geo& g, const 
                                               const double* __restrict__ x, double* __restrict__ r,
                                               T* __restrict__ b0, const double* __restrict__ invd,
                                               double* __restrict__ z, double* __restrict__ p,
-                                              const State* __restrict__ st, double* __restrict__ part) {
+                                              const State* __restrict__ st, double* __restrict__ part,
+                                              double* __restrict__ bw = nullptr) {
     if (MODE != RES_CHECK && st->done) return;
     const int q = B.x * blockDim.x + threadIdx.x;
     const int j = B.y * blockDim.y + threadIdx.y;
@@ -53,12 +56,13 @@ __device__ __forceinline__ void residual_body(const Blk& B, const Geo& g, const 
         const Col c = make_col(g, i, j);
         for (int l = 0; l < g.nz; ++l) {
             const long id = (long)l * g.plane + c.c0;
-            const double bi = b[id];
+            const double bi = MODE == RES_RHS ? b[id] + (double)o.cap[id] * inv_dt * x[id] : b[id];
+            if (MODE == RES_RHS) bw[id] = bi;
             double ri = bi;
             if (!xz) ri = bi - flux_apply_cell(g, o.gE, o.gN, o.gU, o.gB, o.cap, x, c, l, inv_dt);
             v[0] += ri * ri;
             v[1] += bi * bi;
-            if (MODE == RES_MG) {
+            if (MODE == RES_MG || MODE == RES_RHS) {
                 r[id] = ri;
                 b0[id] = (T)ri;
             } else if (MODE == RES_JACOBI) {
@@ -77,8 +81,9 @@ __global__ __launch_bounds__(1024) void k_residual(Geo g, Ops<T> o, const double
                                                    const double* __restrict__ x, double* __restrict__ r,
                                                    T* __restrict__ b0, const double* __restrict__ invd,
                                                    double* __restrict__ z, double* __restrict__ p,
-                                                   const State* __restrict__ st, double* __restrict__ part) {
-    residual_body<T, MODE>(hw_blk(), g, o, b, x, r, b0, invd, z, p, st, part);
+                                                   const State* __restrict__ st, double* __restrict__ part,
+                                                   double* __restrict__ bw) {
+    residual_body<T, MODE>(hw_blk(), g, o, b, x, r, b0, invd, z, p, st, part, bw);
 }
 
 // residual_body for a level with column types (k_residual_typed): the same
@@ -91,7 +96,7 @@ template <class T, int NZ, int MODE, bool TR, bool UNI, bool XZ>
 __device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<T>& y, const double* __restrict__ b,
                                                       const double* __restrict__ x, double* __restrict__ r,
                                                       T* __restrict__ b0, const Col& c, int t, double inv_dt,
-                                                      double (&v)[3]) {
+                                                      double (&v)[3], double* __restrict__ bw) {
     const int nz = g.nz;
     if (XZ) {
 #pragma unroll
@@ -118,7 +123,8 @@ __device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<
         const bool up = l < nz - 1;
         const long base = (long)lc * g.plane, id = base + c.c0;
         const TypeRow<T> rw = UNI ? y.urow(t, lc) : y.row(t, lc);
-        const double bi = b[id];
+        // RES_RHS: k_rhs_transient's b, with C from the type (the same float)
+        const double bi = MODE == RES_RHS ? b[id] + (double)rw.v[TF_C] * inv_dt * xc : b[id];
         const double xu = x[id + (up ? g.plane : 0)];
         const double xw = x[base + c.pw], xe = x[base + c.pe], xs = x[base + c.ps], xn = x[base + c.pn];
         const double Gw = c.hw ? (double)rw.v[TF_W] : 0.0, Ge = c.he ? (double)rw.v[TF_E] : 0.0;
@@ -136,7 +142,8 @@ __device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<
         if (plane_live<NZ>(l, nz)) {
             v[0] += ri * ri;
             v[1] += bi * bi;
-            if (MODE == RES_MG) {
+            if (MODE == RES_RHS) bw[id] = bi;
+            if (MODE == RES_MG || MODE == RES_RHS) {
                 r[id] = ri;
                 b0[id] = (T)ri;
             }
@@ -150,7 +157,7 @@ template <class T, int NZ, int MODE>
 __global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
                                                          const double* __restrict__ x, double* __restrict__ r,
                                                          T* __restrict__ b0, const State* __restrict__ st,
-                                                         double* __restrict__ part) {
+                                                         double* __restrict__ part, double* __restrict__ bw) {
     if (MODE != RES_CHECK && st->done) return;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -163,13 +170,13 @@ __global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const
         const int ty = y.ty[c.c0];
         const int rep = wave_rep_type(ty);
         if (xz)
-            residual_column_typed<T, NZ, MODE, false, true, true>(g, y, b, x, r, b0, c, 0, inv_dt, v);
+            residual_column_typed<T, NZ, MODE, false, true, true>(g, y, b, x, r, b0, c, 0, inv_dt, v, bw);
         else if (__all(ty == rep)) {
-            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v);
-            else residual_column_typed<T, NZ, MODE, false, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v);
+            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v, bw);
+            else residual_column_typed<T, NZ, MODE, false, true, false>(g, y, b, x, r, b0, c, rep, inv_dt, v, bw);
         } else {
-            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v);
-            else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v);
+            if (inv_dt != 0.0) residual_column_typed<T, NZ, MODE, true, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v, bw);
+            else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v, bw);
         }
     }
     block_partials<3>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
@@ -794,6 +801,41 @@ __global__ __launch_bounds__(kVecThreads) void k_energy(Geo g, Ops<T> o, const d
     const long stride = (long)gridDim.x * blockDim.x;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         const double xv = x[k];
+        const long q = k % g.plane;
+        int i;
+        const bool cell = q < g.slots() && slot_column(g, (int)(q % (2 * g.hx)), (int)(q / (2 * g.hx)), i);
+        v[0] += (double)o.cap[k] * xv;
+        if (k >= top) v[1] += (double)o.gU[k] * xv;
+        if (k < g.plane) v[1] += (double)o.gB[k] * xv;
+        if (cell) v[2] = fmax(v[2], xv);
+    }
+    block_partials<3, 4u>(v, part);
+}
+
+// The end of a backward-Euler step in one pass: k_finish (the lagged
+// x += alpha p, or x = 0 when b was 0) and k_energy of the completed x, each
+// cell's terms added in k_energy's order by the same thread, so the partials
+// are k_energy's. Saves the second pass over x and a launch per step.
+template <class T, class PT>
+__global__ __launch_bounds__(kVecThreads) void k_finish_energy(Geo g, Ops<T> o, double* __restrict__ x,
+                                                        const PT* __restrict__ p, const State* __restrict__ st,
+                                                        double* __restrict__ part) {
+    double v[3] = {0.0, 0.0, -1.0e300};
+    const long n = g.cells(), top = (long)(g.nz - 1) * g.plane;
+    const long stride = (long)gridDim.x * blockDim.x;
+    const int zero = st->zero_x, lag = st->xlag;
+    const double alpha = st->alpha;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double xv;
+        if (zero) {
+            xv = 0.0;
+            x[k] = xv;
+        } else if (lag) {
+            xv = x[k] + alpha * (double)p[k];
+            x[k] = xv;
+        } else {
+            xv = x[k];
+        }
         const long q = k % g.plane;
         int i;
         const bool cell = q < g.slots() && slot_column(g, (int)(q % (2 * g.hx)), (int)(q / (2 * g.hx)), i);

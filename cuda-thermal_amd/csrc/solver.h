@@ -698,14 +698,16 @@ struct Solver final : SolverBase {
     // r = b - (A + C/dt) x with b0 = (T) r (RES_MG), or the steady ||P - A v||
     // (RES_CHECK, no writes), nblk_res partials into pp: the typed column
     // walk (64 x 4 blocks) when the fine level has column types.
+    // RES_RHS: bb is P and the step's b = P + C/dt x is written to bw.
     template <int MODE>
-    int launch_residual(const double* bb, const double* xx, double* pp) {
+    int launch_residual(const double* bb, const double* xx, double* pp, double* bw = nullptr) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
-        double* rr = MODE == RES_MG ? r : nullptr;
-        T* b0 = MODE == RES_MG ? L[0].b : nullptr;
+        double* rr = MODE == RES_MG || MODE == RES_RHS ? r : nullptr;
+        T* b0 = MODE == RES_MG || MODE == RES_RHS ? L[0].b : nullptr;
         if (L[0].ntypes) {
             const dim3 tg = col_grid(2 * g0.hx, g0.ny);
-#define TH_RES(NZ) k_residual_typed<T, NZ, MODE><<<tg, col_block(), 0, s>>>(g0, L[0].typed(), bb, xx, rr, b0, st, pp)
+#define TH_RES(NZ) \
+    k_residual_typed<T, NZ, MODE><<<tg, col_block(), 0, s>>>(g0, L[0].typed(), bb, xx, rr, b0, st, pp, bw)
             if (g0.nz <= 4) TH_RES(4);
             else if (g0.nz <= 8) TH_RES(8);
             else if (g0.nz <= 12) TH_RES(12);
@@ -717,19 +719,22 @@ struct Solver final : SolverBase {
 #undef TH_RES
         } else {
             k_residual<T, MODE><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), bb, xx, rr, b0, nullptr, nullptr,
-                                                                 nullptr, st, pp);
+                                                                 nullptr, st, pp, bw);
         }
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
 
-    // Start: residual, first preconditioner application, rz, p = z.
-    int enqueue_start() {
+    // Start: residual, first preconditioner application, rz, p = z. rhs: a
+    // backward-Euler step's b = P + C/dt x formed by the residual kernel
+    // itself (multigrid).
+    int enqueue_start(bool rhs = false) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const unsigned vb = vec_blocks(n0);
         int rc;
         if (mg()) {
-            if ((rc = launch_residual<RES_MG>(b, x, part_res))) return rc;
+            if ((rc = rhs ? launch_residual<RES_RHS>(P, x, part_res, b) : launch_residual<RES_MG>(b, x, part_res)))
+                return rc;
             if ((rc = vcycle(0, Cons{part_res, (int)nblk_res, CONS_START}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
             HIPTRY(hipGetLastError());
@@ -738,7 +743,7 @@ struct Solver final : SolverBase {
             return TH_OK;
         }
         k_residual<T, RES_JACOBI><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
-                                                                   part_res);
+                                                                   part_res, nullptr);
         HIPTRY(hipGetLastError());
         return finalize<3, FIN_RES_JACOBI>(nblk_stencil, part_res);
     }
@@ -971,6 +976,9 @@ struct Solver final : SolverBase {
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = enqueue_factors())) return rc;
+        // the plain warm start: b = P + C/dt x inside the start's residual
+        // kernel (one pass over x and P less than k_rhs_transient + residual)
+        if (predictor == 0 && mg()) return enqueue_start(true);
         if ((rc = set_rhs_transient())) return rc;
         if (predictor == 1 || predictor == 3) {
             k_predict<<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, xprev, st);
@@ -1001,9 +1009,7 @@ struct Solver final : SolverBase {
     }
     int enqueue_step_tail() {
         const unsigned vb = vec_blocks(n0);
-        k_finish<T><<<vb, kVecThreads, 0, s>>>(n0, x, pm, st);
-        HIPTRY(hipGetLastError());
-        k_energy<T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, part);
+        k_finish_energy<T, T><<<vb, kVecThreads, 0, s>>>(g0, L[0].ops(), x, pm, st, part);
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = finalize<3, FIN_ENERGY, 4u>(vb))) return rc;
