@@ -8,6 +8,7 @@ progress goes to stderr.
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--n 1024] [--no-cpu]
   python bench.py --kernels-only   # the hot kernels only, for rocprofv3 --pmc
+  python bench.py --trace-leg transient|learned   # one short leg, for kernel traces
 """
 import argparse
 import json
@@ -348,6 +349,36 @@ def kernels_only(n):
         log(f"{which}: {ms * 1e3:.1f} us, {nb / 1e6:.1f} MB algorithmic, {nb / ms / 1e6:.0f} GB/s")
 
 
+def trace_leg(which, n, seed):
+    """A short run of one leg, for rocprofv3 --kernel-trace: `transient`, 3
+    + 12 backward-Euler steps of the mixed solver on the n^2 x 16 TSV stack
+    (the 100-step leg's first steps); `learned`, 4 mixed solves with the
+    surrogate's start (th_solver_set_x_learned)."""
+    folder, model = make_case(n, seed)
+    s = T.Solver(model, precision="mixed", true_residual=(which == "learned"))
+    if which == "transient":
+        kw = dict(dt0=1e-5, growth=1.1, dt_max=1e30, t_end=1e30, tol=1e-9)
+        T.run_transient(s, **dict(kw, max_steps=3))
+        g = T.run_transient(s, **dict(kw, max_steps=12))
+        log(f"transient: 12 steps in {g['solve_ms']:.2f} ms, iterations {[int(k) for k in g['iters']]}")
+    else:
+        from thermal_amd import surrogate as TS
+
+        sur = TS.load()
+        dfeats = T.DeviceBuffer(TS.features(W.config2_spec(seed), TS.GRID))
+        zf = TS.plane_centres(W.config2_layers())
+        for _ in range(4):
+            s.set_x_zero()
+            t0 = time.perf_counter()
+            s.set_x_learned(sur, dfeats.ptr, TS.COARSE_Z, zf, scale=T.Surrogate.T_SCALE, passes=1)
+            s.set_rhs_steady()
+            st = s.solve(TOL, 500)
+            log(f"learned: {(time.perf_counter() - t0) * 1e3:.2f} ms, {st['iters']} iterations, "
+                f"guess kept: {s.lifted_stats()[0]}")
+        sur.close()
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,6 +388,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
+    ap.add_argument("--trace-leg", choices=("transient", "learned"), help="one short leg, for kernel traces")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -368,6 +400,9 @@ def main():
         sys.exit(2)
     if args.kernels_only:
         kernels_only(args.n)
+        return
+    if args.trace_leg:
+        trace_leg(args.trace_leg, args.n, args.seed)
         return
 
     n = args.n
