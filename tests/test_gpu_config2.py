@@ -7,8 +7,8 @@ Two oracles per case:
   restriction, V-cycle in the solver's precision): the same arithmetic, so
   the iteration count within one and the field within 1e-8 relative;
 * the reference's plain FP64 algorithm (FP64 V-cycle, Fletcher-Reeves beta,
-  residual-form restriction, pcg3d_cpu.cpp:115-185 with -DMG_DOUBLE): the
-  north star's bar of 0.0005 C max-abs.
+  residual-form restriction, pcg3d_cpu.cpp:115-185 with -DMG_DOUBLE): at
+  most 0.0005 C max-abs apart.
 Every result also meets a true relative residual <= 1e-10 (flux form) and
 energy balance (heat out = power in) to 1e-6.
 
