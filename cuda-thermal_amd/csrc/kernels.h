@@ -79,6 +79,7 @@ struct State {
     double inv_dt;  // 1/dt of the current operator (0 = steady)
     double tol;
     double bb, rr, rz, rzo, pAp, alpha, beta, relres;
+    double relres0;  // relres at the start of the solve (its first residual)
     double rzb[2];                 // r.z by iteration parity (multigrid path)
     double chk_rr, chk_bb;         // true-residual check
     double e_heat, e_out, e_tmax;  // energy: sum C T, heat out, max T
@@ -349,6 +350,7 @@ __device__ inline bool consume_convergence(const double* __restrict__ part, int 
             st->rr = tot[0];
             st->rzo = 0.0;
             st->relres = relres;
+            st->relres0 = relres;
             st->zero_x = bb == 0.0;
             st->converged = fin;
             st->done = fin;

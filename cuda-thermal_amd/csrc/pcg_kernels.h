@@ -540,12 +540,14 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const double* __restri
         st->rzo = 0.0;
         if (tot[1] == 0.0) {  // b = 0: x = 0 is the answer
             st->relres = 0.0;
+            st->relres0 = 0.0;
             st->done = 1;
             st->converged = 1;
             st->zero_x = 1;
             return;
         }
         st->relres = sqrt(tot[0] / tot[1]);
+        st->relres0 = st->relres;
         if (st->relres < st->tol) {
             st->done = 1;
             st->converged = 1;

@@ -872,7 +872,10 @@ struct Solver final : SolverBase {
     // the true-residual check, tol < 1e-8); the batch before it stops where
     // the replacement is due. After an iteration x is complete (the p update
     // applies the lagged x += alpha p).
-    int iterate(int maxit, int first = 1, bool replace = false) {
+    // fresh: the solve's start was queued without the host looking at it
+    // (h_st holds k_begin's fields only): the first batch's rate is taken
+    // from the start's residual, relres0.
+    int iterate(int maxit, int first = 1, bool replace = false, bool fresh = false) {
         int rc;
         double rate = 0.0;  // relres reduction per iteration, once known
         replace = replace && opt.true_residual && mg() && h_st->tol < 1e-8;
@@ -895,11 +898,13 @@ struct Solver final : SolverBase {
             }
             k = std::min(k, maxit - h_st->iters);
             const int it0 = h_st->iters;
-            const double rel0 = h_st->relres;
+            double rel0 = h_st->relres;
             for (int q = 0; q < k - 1; ++q)
                 if ((rc = run_iter())) return rc;
             if ((rc = run_iter_last())) return rc;
             if ((rc = fetch_state())) return rc;
+            if (fresh) rel0 = h_st->relres0;
+            fresh = false;
             const int did = h_st->iters - it0;
             if (did > 0 && rel0 > 0.0 && h_st->relres > 0.0) rate = std::pow(h_st->relres / rel0, 1.0 / did);
             if (did == 0) break;  // nothing ran: the device is done
@@ -916,8 +921,21 @@ struct Solver final : SolverBase {
         xz_device = false;
         k_begin<<<1, 1, 0, s>>>(st, tol, maxit, xz);
         HIPTRY(hipGetLastError());
-        if ((rc = run_start()) || (rc = fetch_state())) return rc;
-        if ((rc = converge(maxit, 1))) return rc;
+        if ((rc = run_start())) return rc;
+        // The first iteration is queued behind the start without a look at
+        // it (a start that already meets tol leaves it returning at once);
+        // the host's mirror gets k_begin's fields meanwhile, and the batches
+        // that follow are sized as before from the start's residual and the
+        // first iteration's (the batch ends decide where the residual
+        // replacement falls: a first batch of two moved it and cost an
+        // iteration at 1024^2 x 16).
+        h_st->tol = tol;
+        h_st->maxit = maxit;
+        h_st->iters = 0;
+        h_st->done = 0;
+        h_st->converged = 0;
+        h_st->zero_x = 0;
+        if ((rc = converge(maxit, 1, true))) return rc;
         k_finish<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, pm, st);
         HIPTRY(hipGetLastError());
         if (out) {
@@ -932,9 +950,10 @@ struct Solver final : SolverBase {
     // true-residual check: the recurrence ||r|| drifts from ||b - A x|| by
     // rounding; restart from x with the true residual until that one is
     // below tol too.
-    int converge(int maxit, int first) {
+    int converge(int maxit, int first, bool fresh = false) {
         int rc;
-        if (maxit > 0 && (rc = iterate(maxit, first, true))) return rc;
+        if (maxit > 0 && (rc = iterate(maxit, first, true, fresh))) return rc;
+        if (maxit <= 0 && fresh && (rc = fetch_state())) return rc;  // the start's own state
         for (int restart = 0; opt.true_residual && mg() && h_st->converged && !h_st->zero_x && restart < 4; ++restart) {
             k_finish<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, x, pm, st);  // x complete
             HIPTRY(hipGetLastError());
