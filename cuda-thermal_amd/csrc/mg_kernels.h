@@ -319,9 +319,10 @@ __global__ __launch_bounds__(256) void k_restrict(Geo f, Ops<T> o, const T* __re
 // waited for its own: 30 us per launch on the 128^2 ... 16^2 levels at
 // nz = 16 whatever their size. The same arithmetic in the same order.
 template <class T, int NZ>
-__global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__ finv, const State* __restrict__ st) {
-    const int q = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * kColsY + threadIdx.y;
+__device__ __forceinline__ void factor_body(const Geo& g, const Ops<T>& o, T* __restrict__ finv,
+                                            const State* __restrict__ st, int bx, int by) {
+    const int q = bx * kColsX + threadIdx.x;
+    const int j = by * kColsY + threadIdx.y;
     int i;
     if (!st->refactor || j >= g.ny || q >= 2 * g.hx || !slot_column(g, q, j, i)) return;
     const Col c = make_col(g, i, j);
@@ -357,6 +358,30 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
         }
 }
 
+// The factors of up to kFactorLevels levels in one launch (every level of a
+// hierarchy has the same plane count): a flat grid whose blocks find their
+// level in blk0 and their place in its own 64 x 4-block grid. One launch
+// instead of one per level for the small levels, whose launches cost the
+// same whatever their size.
+constexpr int kFactorLevels = 16;
+template <class T>
+struct FactorLevels {
+    int n;
+    int blk0[kFactorLevels + 1];  // first flat block of each level; blk0[n]: all blocks
+    int gx[kFactorLevels];        // blocks along a row of each level's grid
+    Geo g[kFactorLevels];
+    Ops<T> o[kFactorLevels];
+    T* finv[kFactorLevels];
+};
+template <class T, int NZ>
+__global__ __launch_bounds__(256) void k_factor_levels(FactorLevels<T> L, const State* __restrict__ st) {
+    const int b = blockIdx.x;
+    int k = 0;
+    while (k + 1 < L.n && b >= L.blk0[k + 1]) ++k;
+    const int local = b - L.blk0[k];
+    factor_body<T, NZ>(L.g[k], L.o[k], L.finv[k], st, local % L.gx[k], local / L.gx[k]);
+}
+
 // ---- typed fine level ----------------------------------------------------------
 // The fine level of a layered stack has few distinct columns (see levels.h:
 // column_types): each plane slot holds a 2-byte type and the faces come from
@@ -368,9 +393,8 @@ __global__ __launch_bounds__(256) void k_factor(Geo g, Ops<T> o, T* __restrict__
 // 1/pivot of every type for the current dt (k_factor's arithmetic), every
 // plane's row read before the recurrence (NZ: the plane-count bucket).
 template <class T, int NZ>
-__global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const unsigned char* __restrict__ fl, int nt,
-                                                     int nz, int nzp, const State* __restrict__ st) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void factor_types_body(T* __restrict__ tab, const unsigned char* __restrict__ fl, int nt,
+                                                  int nz, int nzp, const State* __restrict__ st, int t) {
     if (!st->refactor || t >= nt) return;
     T* rows = tab + (long)t * nzp * kTypeFields;  // rows[l * kTypeFields + field]
     const T gB = rows[TF_B];
@@ -404,6 +428,24 @@ __global__ __launch_bounds__(64) void k_factor_types(T* __restrict__ tab, const 
             rows[l * kTypeFields + TF_FI] = inv;
             cp_prev = upper * inv;
         }
+}
+
+// The types of up to kFactorLevels levels in one launch: block b takes 64
+// types of the level whose range of blocks holds it.
+template <class T>
+struct TypeLevels {
+    int n;
+    int blk0[kFactorLevels + 1];
+    int nt[kFactorLevels];
+    T* tab[kFactorLevels];
+    const unsigned char* fl[kFactorLevels];
+};
+template <class T, int NZ>
+__global__ __launch_bounds__(64) void k_factor_types_levels(TypeLevels<T> L, int nz, const State* __restrict__ st) {
+    const int b = blockIdx.x;
+    int k = 0;
+    while (k + 1 < L.n && b >= L.blk0[k + 1]) ++k;
+    factor_types_body<T, NZ>(L.tab[k], L.fl[k], L.nt[k], nz, nz, st, (b - L.blk0[k]) * 64 + (int)threadIdx.x);
 }
 
 // sweep_column with the faces from the column's type. UNI: every column of

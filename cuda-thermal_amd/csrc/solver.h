@@ -382,37 +382,67 @@ struct Solver final : SolverBase {
     int enqueue_factors() {
         if (mg()) {
             // a level with column types reads its factors from the type table
-            // only: its per-cell 1/pivot is never read
-            for (DevLevel<T>& d : L) {
-                if (d.ntypes) continue;
-                const dim3 grid = col_grid(2 * d.g.hx, d.g.ny);
-#define TH_FACTOR(NZ) k_factor<T, NZ><<<grid, col_block(), 0, s>>>(d.g, d.ops(), d.finv, st)
-                if (d.g.nz <= 4) TH_FACTOR(4);
-                else if (d.g.nz <= 8) TH_FACTOR(8);
-                else if (d.g.nz <= 12) TH_FACTOR(12);
-                else if (d.g.nz <= 16) TH_FACTOR(16);
-                else if (d.g.nz <= 24) TH_FACTOR(24);
-                else if (d.g.nz <= 32) TH_FACTOR(32);
-                else if (d.g.nz <= 48) TH_FACTOR(48);
-                else TH_FACTOR(64);
+            // only: its per-cell 1/pivot is never read. The other levels'
+            // factors, and the typed levels' types, one launch each per
+            // kFactorLevels levels (every level has the fine level's nz).
+            FactorLevels<T> fl{};
+            TypeLevels<T> tl{};
+            auto flush = [&]() -> int {
+                if (fl.n) {
+                    const unsigned nb = (unsigned)fl.blk0[fl.n];
+#define TH_FACTOR(NZ) k_factor_levels<T, NZ><<<nb, col_block(), 0, s>>>(fl, st)
+                    if (g0.nz <= 4) TH_FACTOR(4);
+                    else if (g0.nz <= 8) TH_FACTOR(8);
+                    else if (g0.nz <= 12) TH_FACTOR(12);
+                    else if (g0.nz <= 16) TH_FACTOR(16);
+                    else if (g0.nz <= 24) TH_FACTOR(24);
+                    else if (g0.nz <= 32) TH_FACTOR(32);
+                    else if (g0.nz <= 48) TH_FACTOR(48);
+                    else TH_FACTOR(64);
 #undef TH_FACTOR
-                HIPTRY(hipGetLastError());
-            }
-            for (DevLevel<T>& d : L)
-                if (d.ntypes) {
-                    const unsigned nb = (unsigned)((d.ntypes + 63) / 64);
-#define TH_FTYPES(NZ) k_factor_types<T, NZ><<<nb, 64, 0, s>>>(d.tab, d.tfl, d.ntypes, d.g.nz, d.g.nz, st)
-                    if (d.g.nz <= 4) TH_FTYPES(4);
-                    else if (d.g.nz <= 8) TH_FTYPES(8);
-                    else if (d.g.nz <= 12) TH_FTYPES(12);
-                    else if (d.g.nz <= 16) TH_FTYPES(16);
-                    else if (d.g.nz <= 24) TH_FTYPES(24);
-                    else if (d.g.nz <= 32) TH_FTYPES(32);
-                    else if (d.g.nz <= 48) TH_FTYPES(48);
+                    HIPTRY(hipGetLastError());
+                    fl.n = 0;
+                }
+                if (tl.n) {
+                    const unsigned nb = (unsigned)tl.blk0[tl.n];
+#define TH_FTYPES(NZ) k_factor_types_levels<T, NZ><<<nb, 64, 0, s>>>(tl, g0.nz, st)
+                    if (g0.nz <= 4) TH_FTYPES(4);
+                    else if (g0.nz <= 8) TH_FTYPES(8);
+                    else if (g0.nz <= 12) TH_FTYPES(12);
+                    else if (g0.nz <= 16) TH_FTYPES(16);
+                    else if (g0.nz <= 24) TH_FTYPES(24);
+                    else if (g0.nz <= 32) TH_FTYPES(32);
+                    else if (g0.nz <= 48) TH_FTYPES(48);
                     else TH_FTYPES(64);
 #undef TH_FTYPES
                     HIPTRY(hipGetLastError());
+                    tl.n = 0;
                 }
+                return TH_OK;
+            };
+            int rc;
+            for (DevLevel<T>& d : L) {
+                if (d.ntypes) {
+                    if (tl.n == kFactorLevels && (rc = flush())) return rc;
+                    const int k = tl.n++;
+                    tl.blk0[k] = k ? tl.blk0[k] : 0;
+                    tl.nt[k] = d.ntypes;
+                    tl.tab[k] = d.tab;
+                    tl.fl[k] = d.tfl;
+                    tl.blk0[k + 1] = tl.blk0[k] + (d.ntypes + 63) / 64;
+                } else {
+                    if (fl.n == kFactorLevels && (rc = flush())) return rc;
+                    const dim3 grid = col_grid(2 * d.g.hx, d.g.ny);
+                    const int k = fl.n++;
+                    fl.blk0[k] = k ? fl.blk0[k] : 0;
+                    fl.gx[k] = (int)grid.x;
+                    fl.g[k] = d.g;
+                    fl.o[k] = d.ops();
+                    fl.finv[k] = d.finv;
+                    fl.blk0[k + 1] = fl.blk0[k] + (int)(grid.x * grid.y);
+                }
+            }
+            if ((rc = flush())) return rc;
         } else {
             const DevLevel<T>& d = L[0];
             k_invdiag<T><<<col_grid(2 * g0.hx, g0.ny), col_block(), 0, s>>>(g0, d.ops(), invd, st);
