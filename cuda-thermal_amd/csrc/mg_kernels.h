@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_factor_levels(FactorLevels<T> L, const 
 // are built), and every kernel below does its dense counterpart's arithmetic
 // in the same order: results are identical.
 
-// 1/pivot of every type for the current dt (k_factor's arithmetic), every
+// 1/pivot of every type for the current dt (factor_body's arithmetic), every
 // plane's row read before the recurrence (NZ: the plane-count bucket).
 template <class T, int NZ>
 __device__ __forceinline__ void factor_types_body(T* __restrict__ tab, const unsigned char* __restrict__ fl, int nt,

@@ -994,7 +994,7 @@ struct Solver final : SolverBase {
             // graph (a V-cycle's worth of launches) is not replayed. The
             // kernel writes nothing (RES_TRUE): the start graph of a restart
             // computes r and the V-cycle's b itself. (With RES_MG's writes
-            // the test took 135 us at 1024^2 x 16, without them 83.)
+            // the test took 135 us at 1024^2 x 16, without them 102.)
             if ((rc = launch_residual<RES_TRUE>(b, x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
                 (rc = fetch_state()))
                 return rc;
