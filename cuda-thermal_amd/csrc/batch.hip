@@ -32,6 +32,8 @@ struct LevelView {
     const float* finv;
     float* x;
     float* b;
+    Typed<float> y;  // column types (ntypes > 0): the sweeps read them instead of o and finv
+    int ntypes;
 };
 
 struct Inst {
@@ -84,8 +86,11 @@ __global__ __launch_bounds__(512) void kb_sweep(const Inst* __restrict__ I, cons
     const LevelView& C = in.L[k + 1 < in.nlev ? k + 1 : k];
     const double* cp = cons == CONS_START ? in.part_res : in.part_upd;
     const int cn = cons == CONS_START ? in.nblk_stencil : in.nblk_vec;
-    sweep_body<float, NZ, MODE>(B, L.g, L.o, L.finv, L.b, L.x, colour, C.g, C.x, in.st, cp, cn,
-                                k == 0 ? cons : (int)CONS_NONE);
+    const int cm = k == 0 ? cons : (int)CONS_NONE;
+    if (L.ntypes)
+        sweep_typed_body<float, NZ, MODE, false>(B, L.g, L.y, L.b, L.x, colour, C.g, C.x, in.st, cp, cn, cm);
+    else
+        sweep_body<float, NZ, MODE>(B, L.g, L.o, L.finv, L.b, L.x, colour, C.g, C.x, in.st, cp, cn, cm);
 }
 
 __global__ __launch_bounds__(512) void kb_restrict(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab,
@@ -260,7 +265,6 @@ struct Batch {
         for (int i = 0; i < n; ++i) {
             Solver<float>* x = new Solver<float>();
             S.push_back(x);
-            x->allow_types = false;  // the batched kernels read the per-cell arrays
             int rc = x->init(*m[i], io);
             if (rc) return rc;
             k_set_flexible<<<1, 1, 0, x->s>>>(x->st, 1);
@@ -296,11 +300,14 @@ struct Batch {
             d.nlev = (int)x.L.size();
             d.tail_k0 = (int)x.tail_k0;
             for (size_t k = 0; k < x.L.size(); ++k)
-                d.L[k] = LevelView{x.L[k].g, x.L[k].ops(), x.L[k].finv, x.L[k].x, x.L[k].b};
+                d.L[k] = LevelView{x.L[k].g, x.L[k].ops(), x.L[k].finv, x.L[k].x, x.L[k].b, x.L[k].typed(),
+                                   x.L[k].ntypes};
             Tail<float>& t = hT[i];
             t.n = d.nlev - d.tail_k0;
             for (int k = 0; k < t.n; ++k) {
                 const DevLevel<float>& L = x.L[d.tail_k0 + k];
+                // the tail reads per-cell factors, which typed levels do not have
+                if (L.ntypes) return fail(TH_ERR_ARG, "batched solver: a typed level in the LDS tail");
                 t.L[k] = TailLevel<float>{L.g, L.ops(), L.finv, L.x, L.b};
             }
             lds = std::max(lds, x.lds_bytes);

@@ -504,15 +504,18 @@ __device__ __forceinline__ void sweep_column_typed(const Geo& g, const Typed<T>&
     for (int l = 0; l < NZ; ++l)
         if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
 }
+// Block (B.x, B.y) covers blockDim.x columns of the colour in each of
+// blockDim.y rows (sweep_body's mapping; the batched engine runs it with its
+// own block shapes: the type tests are per wave, whatever rows it spans).
 template <class T, int NZ, int MODE, bool C3>
-__device__ __forceinline__ void sweep_typed_body(const Geo& g, const Typed<T>& y, const T* __restrict__ b,
-                                                 T* __restrict__ x, int colour, const Geo& gc,
+__device__ __forceinline__ void sweep_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
+                                                 const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
                                                  const T* __restrict__ xc, State* st,
                                                  const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
     if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
-    const int t = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
+    const int t = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     if (j >= g.ny) return;
     const int i = 2 * t + ((j + colour) & 1);
     if (i >= g.nx) return;
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T*
                                                      int colour, Geo gc, const T* __restrict__ xc, State* st,
                                                      const double* __restrict__ cons_part, int cons_nblk,
                                                      int cons_mode) {
-    sweep_typed_body<T, NZ, MODE, false>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+    sweep_typed_body<T, NZ, MODE, false>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 // The FP32 prolongation sweep: three coarse loads per plane and at least 4
 // waves per SIMD. With every plane's loads unconditional the compiler would
@@ -542,7 +545,7 @@ template <class T, int NZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_typed_prolong3(
     Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc, const T* __restrict__ xc,
     State* st, const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
-    sweep_typed_body<T, NZ, SW_PROLONG, true>(g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+    sweep_typed_body<T, NZ, SW_PROLONG, true>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 // FP32 plain and prolongation sweeps (NZ <= 16) with every load that does

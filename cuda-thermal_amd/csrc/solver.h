@@ -129,7 +129,7 @@ struct Solver final : SolverBase {
     double *P = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
     double *z = nullptr, *invd = nullptr, *xnat = nullptr;
     T* pm = nullptr;  // multigrid search direction, in the V-cycle's precision (p itself in FP64)
-    bool allow_types = true;  // column types; the batched engine, with kernels of its own, turns them off
+    bool allow_types = true;  // column types on the levels that run the column walk
     double* part = nullptr;
     State* st = nullptr;
     State* h_st = nullptr;  // pinned mirror
