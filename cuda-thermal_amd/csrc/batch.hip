@@ -82,22 +82,34 @@ __global__ __launch_bounds__(512) void kb_sweep(const Inst* __restrict__ I, cons
                                                 int k, int colour, int cons) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    const LevelView& L = in.L[k];
-    const LevelView& C = in.L[k + 1 < in.nlev ? k + 1 : k];
+    // descriptors copied out of the table: values the stores cannot alias,
+    // so the compiler keeps them in scalar registers instead of reloading
+    const LevelView L = in.L[k];
+    const int kc = k + 1 < in.nlev ? k + 1 : k;
+    const Geo gc = in.L[kc].g;
+    const float* xc = in.L[kc].x;
+    State* st = in.st;
     const double* cp = cons == CONS_START ? in.part_res : in.part_upd;
     const int cn = cons == CONS_START ? in.nblk_stencil : in.nblk_vec;
     const int cm = k == 0 ? cons : (int)CONS_NONE;
     if (L.ntypes)
-        sweep_typed_body<float, NZ, MODE, false>(B, L.g, L.y, L.b, L.x, colour, C.g, C.x, in.st, cp, cn, cm);
+        sweep_typed_body<float, NZ, MODE, false>(B, L.g, L.y, L.b, L.x, colour, gc, xc, st, cp, cn, cm);
     else
-        sweep_body<float, NZ, MODE>(B, L.g, L.o, L.finv, L.b, L.x, colour, C.g, C.x, in.st, cp, cn, cm);
+        sweep_body<float, NZ, MODE>(B, L.g, L.o, L.finv, L.b, L.x, colour, gc, xc, st, cp, cn, cm);
 }
 
 __global__ __launch_bounds__(512) void kb_restrict(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab,
                                                    int k) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    restrict_fast_body<float>(B, in.L[k].g, in.L[k].o, in.L[k].x, in.L[k + 1].g, in.L[k + 1].b, in.st);
+    const LevelView L = in.L[k];  // copies, as in kb_sweep
+    const Geo gc = in.L[k + 1].g;
+    float* bc = in.L[k + 1].b;
+    const State* st = in.st;
+    if (L.ntypes)  // one plane per block (the table's z), as restrict_fast_body
+        restrict_fast_typed_body<float, 1>(B, L.g, L.y, L.x, gc, bc, st);
+    else
+        restrict_fast_body<float>(B, L.g, L.o, L.x, gc, bc, st);
 }
 
 // One workgroup per instance; tails[i] describes instance i's tail levels.
@@ -112,19 +124,45 @@ __global__ __launch_bounds__(kTailLdsThreads) void kb_tail(const Inst* __restric
                              reinterpret_cast<float*>(smem));
 }
 
+// Level-0 stencils. Instances whose level 0 has column types run the typed
+// bodies in blocks of 64 x 4 (as the single-instance solver does: the same
+// partial sums) from a table of their own; the others the per-cell bodies
+// in blocks of 64 x 16.
 __global__ __launch_bounds__(1024) void kb_residual(const Inst* __restrict__ I, const Entry* __restrict__ tab,
                                                     int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    residual_body<float, RES_MG>(B, in.g0, in.L[0].o, in.b, in.x, in.r, in.L[0].b, nullptr, nullptr, nullptr, in.st,
-                                 in.part_res);
+    const Geo g = in.g0;  // copies, as in kb_sweep
+    const Ops<float> o = in.L[0].o;
+    residual_body<float, RES_MG>(B, g, o, in.b, in.x, in.r, in.L[0].b, nullptr, nullptr, nullptr, in.st, in.part_res);
+}
+template <int NZ>
+__global__ __launch_bounds__(256) void kb_residual_typed(const Inst* __restrict__ I, const Entry* __restrict__ tab,
+                                                         int ntab) {
+    Blk B;
+    const Inst& in = locate(I, tab, ntab, B);
+    const Geo g = in.g0;  // copies, as in kb_sweep
+    const Typed<float> y = in.L[0].y;
+    residual_typed_body<float, NZ, RES_MG>(B, g, y, in.b, in.x, in.r, in.L[0].b, in.st, in.part_res, nullptr);
 }
 
 template <int NZ>
 __global__ __launch_bounds__(1024) void kb_apply(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
-    apply_body<float, NZ, float>(B, in.g0, in.L[0].o, in.p, in.Ap, in.st, in.part_apply);
+    const Geo g = in.g0;  // copies, as in kb_sweep
+    const Ops<float> o = in.L[0].o;
+    apply_body<float, NZ, float>(B, g, o, in.p, in.Ap, in.st, in.part_apply);
+}
+// at least 6 waves per SIMD, as k_apply_typed_occ
+template <int NZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_apply_typed(
+    const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
+    Blk B;
+    const Inst& in = locate(I, tab, ntab, B);
+    const Geo g = in.g0;  // copies, as in kb_sweep
+    const Typed<float> y = in.L[0].y;
+    apply_typed_body<float, NZ, float>(B, g, y, in.p, in.Ap, in.st, in.part_apply);
 }
 
 __global__ __launch_bounds__(1024) void kb_update(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
@@ -212,7 +250,7 @@ struct Batch {
     int n = 0, nz = 0, maxk = 0;  // maxk: deepest level with separate kernels + 1
     size_t lds = 0;
     int rows = 4, cols = 64, vthreads = 1024;  // V-cycle blocks: rows x (64 / cols) waves of cols columns
-    Table t_stencil, t_vec;
+    Table t_stencil, t_typed, t_vec;  // level-0 stencils: per-cell instances, typed instances
     std::vector<Table> t_sweep, t_restrict;
     hipGraphExec_t g_start = nullptr, g_iter = nullptr;
 
@@ -238,6 +276,7 @@ struct Batch {
     }
     void free_tables() {
         free_table(t_stencil);
+        free_table(t_typed);
         free_table(t_vec);
         for (Table& t : t_sweep) free_table(t);
         for (Table& t : t_restrict) free_table(t);
@@ -292,7 +331,8 @@ struct Batch {
             d.Ap = x.Ap;
             d.part_res = x.part_res;
             d.part_apply = x.part_apply;
-            d.nblk_stencil = (int)x.nblk_stencil;
+            // partials of the level-0 stencils: blocks of 64 x 4 with types (nblk_apply = nblk_res)
+            d.nblk_stencil = (int)(x.L[0].ntypes ? x.nblk_apply : x.nblk_stencil);
             int rc;
             if ((rc = dalloc(&d.part_upd, (size_t)kMaxVecBlocks * 2)) || (rc = dalloc(&d.part_dot, kMaxVecBlocks)))
                 return rc;
@@ -354,7 +394,10 @@ struct Batch {
         for (int i = 0; i < n; ++i) {
             Inst& d = hInst[i];
             const int sb = (int)((2L * d.g0.hx + kColsX - 1) / kColsX), sr = (d.g0.ny + kStencilRows - 1) / kStencilRows;
-            add(t_stencil, i, sb, sr, 1);
+            const int tr = d.L[0].ntypes ? (d.g0.ny + kColsY - 1) / kColsY : sr;
+            if (sb * tr != d.nblk_stencil)  // the consumers read this many partials
+                return fail(TH_ERR_ARG, "batched solver: level-0 blocks differ from the instance's partials");
+            add(d.L[0].ntypes ? t_typed : t_stencil, i, sb, tr, 1);
             const int vb = (int)std::max(1L, std::min((long)kMaxVecBlocks, (d.n0 + vt - 1) / vt));
             add(t_vec, i, vb, 1, 1);
             d.nblk_vec = vb;
@@ -367,7 +410,7 @@ struct Batch {
         }
         HIPTRY(hipMemcpy(dI, hInst.data(), n * sizeof(Inst), hipMemcpyHostToDevice));
         int rc;
-        if ((rc = upload_table(t_stencil)) || (rc = upload_table(t_vec))) return rc;
+        if ((rc = upload_table(t_stencil)) || (rc = upload_table(t_typed)) || (rc = upload_table(t_vec))) return rc;
         for (int k = 0; k < maxk; ++k)
             if ((rc = upload_table(t_sweep[k])) || (rc = upload_table(t_restrict[k]))) return rc;
         if ((rc = capture(&g_start, true)) || (rc = capture(&g_iter, false))) return rc;
@@ -444,20 +487,35 @@ struct Batch {
         return TH_OK;
     }
     int enqueue_start() {
-        kb_residual<<<t_stencil.blocks, dim3(kColsX, kStencilRows), 0, s>>>(dI, t_stencil.d, (int)t_stencil.h.size());
+        if (t_stencil.blocks)
+            kb_residual<<<t_stencil.blocks, dim3(kColsX, kStencilRows), 0, s>>>(dI, t_stencil.d, (int)t_stencil.h.size());
+        if (t_typed.blocks) {
+#define TB_RS(NZ) kb_residual_typed<NZ><<<t_typed.blocks, dim3(kColsX, kColsY), 0, s>>>(dI, t_typed.d, (int)t_typed.h.size())
+            switch (bucket(nz)) {
+                case 4: TB_RS(4); break;
+                case 8: TB_RS(8); break;
+                case 12: TB_RS(12); break;
+                default: TB_RS(16); break;
+            }
+#undef TB_RS
+        }
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = vcycle(CONS_START)) || (rc = vec(kb_dot)) || (rc = vec(kb_pupdate<1>))) return rc;
         return TH_OK;
     }
     int enqueue_iter() {
-#define TB_AP(NZ) kb_apply<NZ><<<t_stencil.blocks, dim3(kColsX, kStencilRows), 0, s>>>(dI, t_stencil.d, (int)t_stencil.h.size())
-        switch (bucket(nz)) {
-            case 4: TB_AP(4); break;
-            case 8: TB_AP(8); break;
-            case 12: TB_AP(12); break;
-            default: TB_AP(16); break;
-        }
+#define TB_AP(K, TB, ROWS, NZ) K<NZ><<<TB.blocks, dim3(kColsX, ROWS), 0, s>>>(dI, TB.d, (int)TB.h.size())
+#define TB_APZ(K, TB, ROWS)                      \
+    switch (bucket(nz)) {                        \
+        case 4: TB_AP(K, TB, ROWS, 4); break;   \
+        case 8: TB_AP(K, TB, ROWS, 8); break;   \
+        case 12: TB_AP(K, TB, ROWS, 12); break; \
+        default: TB_AP(K, TB, ROWS, 16); break; \
+    }
+        if (t_stencil.blocks) TB_APZ(kb_apply, t_stencil, kStencilRows);
+        if (t_typed.blocks) TB_APZ(kb_apply_typed, t_typed, kColsY);
+#undef TB_APZ
 #undef TB_AP
         HIPTRY(hipGetLastError());
         int rc;

@@ -154,13 +154,14 @@ __device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<
     }
 }
 template <class T, int NZ, int MODE>
-__global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
-                                                         const double* __restrict__ x, double* __restrict__ r,
-                                                         T* __restrict__ b0, const State* __restrict__ st,
-                                                         double* __restrict__ part, double* __restrict__ bw) {
+__device__ __forceinline__ void residual_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
+                                                    const double* __restrict__ b, const double* __restrict__ x,
+                                                    double* __restrict__ r, T* __restrict__ b0,
+                                                    const State* __restrict__ st, double* __restrict__ part,
+                                                    double* __restrict__ bw) {
     if (MODE != RES_CHECK && st->done) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int q = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     double v[3] = {0.0, 0.0, 0.0};  // rr, bb, (rz)
     const double inv_dt = MODE == RES_CHECK ? 0.0 : st->inv_dt;
     const bool xz = MODE == RES_MG && st->xz;
@@ -179,7 +180,14 @@ __global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const
             else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v, bw);
         }
     }
-    block_partials<3>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+    block_partials<3>(v, part, B.flat());
+}
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
+                                                         const double* __restrict__ x, double* __restrict__ r,
+                                                         T* __restrict__ b0, const State* __restrict__ st,
+                                                         double* __restrict__ part, double* __restrict__ bw) {
+    residual_typed_body<T, NZ, MODE>(hw_blk(), g, y, b, x, r, b0, st, part, bw);
 }
 
 // Ap = (A + C/dt) p; partial p.Ap.
@@ -300,12 +308,12 @@ __device__ __forceinline__ double apply_column_typed(const Geo& g, const Typed<T
     return v;
 }
 template <class T, int NZ, class PT>
-__device__ __forceinline__ void apply_typed_body(const Geo& g, const Typed<T>& y, const PT* __restrict__ p,
-                                                 double* __restrict__ Ap, const State* __restrict__ st,
-                                                 double* __restrict__ part) {
+__device__ __forceinline__ void apply_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
+                                                 const PT* __restrict__ p, double* __restrict__ Ap,
+                                                 const State* __restrict__ st, double* __restrict__ part) {
     if (st->done) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int q = B.x * blockDim.x + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     double v[1] = {0.0};
     const double inv_dt = st->inv_dt;
     int i;
@@ -320,7 +328,7 @@ __device__ __forceinline__ void apply_typed_body(const Geo& g, const Typed<T>& y
             v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true, false>(g, y, p, Ap, c, ty, inv_dt)
                                  : apply_column_typed<T, NZ, PT, false, false>(g, y, p, Ap, c, ty, inv_dt);
     }
-    block_partials<1>(v, part, blockIdx.x + gridDim.x * blockIdx.y);
+    block_partials<1>(v, part, B.flat());
 }
 // 64 x 4 blocks. FP64 faces: the compiler's own register choice (106 us at
 // 1024^2 x 16 with the caches flushed, against 116 for 64 x 16 blocks).
@@ -330,13 +338,13 @@ template <class T, int NZ, class PT>
 __global__ __launch_bounds__(256) void k_apply_typed(Geo g, Typed<T> y, const PT* __restrict__ p,
                                                      double* __restrict__ Ap, const State* __restrict__ st,
                                                      double* __restrict__ part) {
-    apply_typed_body<T, NZ, PT>(g, y, p, Ap, st, part);
+    apply_typed_body<T, NZ, PT>(hw_blk(), g, y, p, Ap, st, part);
 }
 template <class T, int NZ, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_apply_typed_occ(
     Geo g, Typed<T> y, const PT* __restrict__ p, double* __restrict__ Ap, const State* __restrict__ st,
     double* __restrict__ part) {
-    apply_typed_body<T, NZ, PT>(g, y, p, Ap, st, part);
+    apply_typed_body<T, NZ, PT>(hw_blk(), g, y, p, Ap, st, part);
 }
 template <class T, int NZ, class PT = double>
 __global__ __launch_bounds__(1024) void k_apply(Geo g, Ops<T> o, const PT* __restrict__ p,

@@ -90,15 +90,23 @@ def test_small_batched_workload_vs_oracle():
 
 def test_full_batched_workload():
     """The benchmark's batched workload itself: 64 instances of 128^2, 256^2
-    or 512^2 x 10 cells (seeded, blocks on with p = 0.6), one autotuned
-    batched solve to 1e-10. Every instance converges with energy balance
-    1e-6 and a true relative residual <= 1e-9; every instance matches a
-    single-instance solve to 1e-10 of its peak rise and takes its iteration
-    count within one; the smallest and the largest instance match the FP64
-    oracle to 5e-4 K."""
+    or 512^2 x 10 cells (seeded, blocks on with p = 0.6; the 256^2 and
+    512^2 instances have column types). With the default shapes (1024-thread
+    vector blocks, as the single-instance solver) every instance equals its
+    single-instance solve bit for bit, in the same iterations. Then one
+    autotuned batched solve to 1e-10 (its vector blocks may sum the dot
+    products in another order): every instance converges with energy balance
+    1e-6 and a true relative residual <= 1e-9, matches the single solve to
+    1e-10 of its peak rise and takes its iteration count within one; the
+    smallest and the largest instance match the FP64 oracle to 5e-4 K."""
     inst = W.config3_instances(count=64)
     models = [W.build(spec, n) for n, spec in inst]
     b = T.Batch(models)
+    r0 = b.solve(1e-10, 200)
+    singles = [single(m, 1e-10) for m in models]
+    for i, (st, xs) in enumerate(singles):
+        assert r0["iters"][i] == st["iters"], i
+        assert np.array_equal(b.get_x(i), xs), i
     tune = b.autotune(2)
     assert tune["cols_per_wave"] in (16, 32, 64)
     r = b.solve(1e-10, 200)
@@ -108,7 +116,7 @@ def test_full_batched_workload():
         x = b.get_x(i)
         P = m.total_power()
         assert abs(m.heat_out(x) - P) / P < 1e-6, i
-        st, xs = single(m, 1e-10)
+        st, xs = singles[i]
         assert abs(int(r["iters"][i]) - st["iters"]) <= 1, (i, r["iters"][i], st["iters"])
         assert np.abs(x - xs).max() <= 1e-10 * np.abs(xs).max() + 1e-12, i
     for i in (int(np.argmin(sizes)), int(np.argmax(sizes))):
