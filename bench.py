@@ -353,7 +353,18 @@ def trace_leg(which, n, seed):
     """A short run of one leg, for rocprofv3 --kernel-trace: `transient`, 3
     + 12 backward-Euler steps of the mixed solver on the n^2 x 16 TSV stack
     (the 100-step leg's first steps); `learned`, 4 mixed solves with the
-    surrogate's start (th_solver_set_x_learned)."""
+    surrogate's start (th_solver_set_x_learned); `batched`, the batched leg's
+    64 instances, autotuned, then 2 batched solves."""
+    if which == "batched":
+        models = [W.build(spec, k) for k, spec in W.config3_instances(count=64)]
+        b = T.Batch(models)
+        log(f"batched: autotune {b.autotune(2)}")
+        for _ in range(2):
+            r = b.solve(TOL, 500)
+            log(f"batched: {r['ms']:.2f} ms, iterations p50 {np.percentile(r['iters'], 50):.0f} "
+                f"max {int(r['iters'].max())}")
+        b.close()
+        return
     folder, model = make_case(n, seed)
     s = T.Solver(model, precision="mixed", true_residual=(which == "learned"))
     if which == "transient":
@@ -388,7 +399,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
-    ap.add_argument("--trace-leg", choices=("transient", "learned"), help="one short leg, for kernel traces")
+    ap.add_argument("--trace-leg", choices=("transient", "learned", "batched"), help="one short leg, for kernel traces")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

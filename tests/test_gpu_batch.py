@@ -46,6 +46,25 @@ def test_batch_equals_single_solves():
         assert np.array_equal(b.get_x(i), x), i
 
 
+def test_typed_16_plane_instances_equal_single_solves():
+    """The 16-plane TSV stack's floorplan at 256^2 (column types on its top
+    levels), at 300 x 212 (as many columns, but the TSV clusters cover cells
+    in part, so no types: per-cell arrays) and at 64^2, in one batch: each
+    instance equals its single solve bit for bit, in the same iterations."""
+    spec = W.config2_spec(1000)
+    ms = [W.build(spec, 256), W.build(spec, 300, 212), W.build(spec, 64)]
+    b = T.Batch(ms)
+    r = b.solve(1e-10, 200)
+    assert r["converged"].all()
+    for i, m in enumerate(ms):
+        s = T.Solver(m)
+        assert (s.column_types() > 0) == (i == 0), i
+        s.close()
+        st, x = single(m, 1e-10)
+        assert r["iters"][i] == st["iters"], i
+        assert np.array_equal(b.get_x(i), x), i
+
+
 def test_convergence_masks():
     """A zero-power instance finishes at the start (x = 0) while the others
     iterate; their results are unaffected."""
