@@ -77,6 +77,9 @@ __device__ __forceinline__ const Inst& locate(const Inst* __restrict__ I, const 
     return I[e.inst];
 }
 
+// Typed levels run the prolongation sweeps up to 16 planes with the
+// type-independent loads first (sweep_typed_lf_body), as the single solves.
+// (The single solves' loads-first pre-smoothing sweep is slower here.)
 template <int NZ, int MODE>
 __global__ __launch_bounds__(512) void kb_sweep(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab,
                                                 int k, int colour, int cons) {
@@ -92,6 +95,12 @@ __global__ __launch_bounds__(512) void kb_sweep(const Inst* __restrict__ I, cons
     const double* cp = cons == CONS_START ? in.part_res : in.part_upd;
     const int cn = cons == CONS_START ? in.nblk_stencil : in.nblk_vec;
     const int cm = k == 0 ? cons : (int)CONS_NONE;
+    if constexpr (MODE == SW_PROLONG && NZ <= 16) {
+        if (L.ntypes) {
+            sweep_typed_lf_body<float, NZ, MODE>(B, L.g, L.y, L.b, L.x, colour, gc, xc, st, cp, cn, cm);
+            return;
+        }
+    }
     if (L.ntypes)
         sweep_typed_body<float, NZ, MODE, false>(B, L.g, L.y, L.b, L.x, colour, gc, xc, st, cp, cn, cm);
     else

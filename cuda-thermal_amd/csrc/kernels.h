@@ -109,6 +109,10 @@ struct State {
 struct Blk {
     int x, y, z, gx, gy, gz;
     __device__ int flat() const { return x + gx * (y + gy * z); }
+    __device__ int flat_xy() const { return x + gx * y; }  // 2D grids (z = 0)
+    // first thread index along x: blocks of BX threads, or blockDim.x (BX = 0)
+    template <int BX>
+    __device__ int x0() const { return BX ? x * BX : x * (int)blockDim.x; }
     __device__ bool first() const { return x == 0 && y == 0 && z == 0; }
 };
 __device__ __forceinline__ Blk hw_blk() {

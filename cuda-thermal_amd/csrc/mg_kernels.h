@@ -507,14 +507,14 @@ __device__ __forceinline__ void sweep_column_typed(const Geo& g, const Typed<T>&
 // Block (B.x, B.y) covers blockDim.x columns of the colour in each of
 // blockDim.y rows (sweep_body's mapping; the batched engine runs it with its
 // own block shapes: the type tests are per wave, whatever rows it spans).
-template <class T, int NZ, int MODE, bool C3>
+template <class T, int NZ, int MODE, bool C3, int BX = 0>
 __device__ __forceinline__ void sweep_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
                                                  const T* __restrict__ b, T* __restrict__ x, int colour, const Geo& gc,
                                                  const T* __restrict__ xc, State* st,
                                                  const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
     if (st->done) return;
     if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
-    const int t = B.x * blockDim.x + threadIdx.x;
+    const int t = B.x0<BX>() + threadIdx.x;
     const int j = B.y * blockDim.y + threadIdx.y;
     if (j >= g.ny) return;
     const int i = 2 * t + ((j + colour) & 1);
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_sweep_typed(Geo g, Typed<T> y, const T*
                                                      int colour, Geo gc, const T* __restrict__ xc, State* st,
                                                      const double* __restrict__ cons_part, int cons_nblk,
                                                      int cons_mode) {
-    sweep_typed_body<T, NZ, MODE, false>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+    sweep_typed_body<T, NZ, MODE, false, kColsX>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 // The FP32 prolongation sweep: three coarse loads per plane and at least 4
 // waves per SIMD. With every plane's loads unconditional the compiler would
@@ -545,7 +545,7 @@ template <class T, int NZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_typed_prolong3(
     Geo g, Typed<T> y, const T* __restrict__ b, T* __restrict__ x, int colour, Geo gc, const T* __restrict__ xc,
     State* st, const double* __restrict__ cons_part, int cons_nblk, int cons_mode) {
-    sweep_typed_body<T, NZ, SW_PROLONG, true>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
+    sweep_typed_body<T, NZ, SW_PROLONG, true, kColsX>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 // FP32 plain and prolongation sweeps (NZ <= 16) with every load that does
@@ -584,18 +584,18 @@ __device__ __forceinline__ void sweep_column_typed_loaded(const Geo& g, const Ty
     for (int l = 0; l < NZ; ++l)
         if (l < nz) x[(long)l * g.plane + c.c0] = s[l];
 }
-template <class T, int NZ, int MODE>
-__global__ __launch_bounds__(256) void k_sweep_typed_lf(Geo g, Typed<T> y, const T* __restrict__ b,
-                                                        T* __restrict__ x, int colour, Geo gc,
-                                                        const T* __restrict__ xc, State* st,
-                                                        const double* __restrict__ cons_part, int cons_nblk,
-                                                        int cons_mode) {
+template <class T, int NZ, int MODE, int BX = 0>
+__device__ __forceinline__ void sweep_typed_lf_body(const Blk& B, const Geo& g, const Typed<T>& y,
+                                                    const T* __restrict__ b, T* __restrict__ x, int colour,
+                                                    const Geo& gc, const T* __restrict__ xc, State* st,
+                                                    const double* __restrict__ cons_part, int cons_nblk,
+                                                    int cons_mode) {
     static_assert(NZ <= 16, "five register columns of NZ values");
     static_assert(MODE == SW_PLAIN || MODE == SW_PROLONG, "plain or prolongation sweep");
     if (st->done) return;
-    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st)) return;
-    const int tt = blockIdx.x * kColsX + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (cons_mode != CONS_NONE && consume_convergence(cons_part, cons_nblk, cons_mode, st, B.first())) return;
+    const int tt = B.x0<BX>() + threadIdx.x;
+    const int j = B.y * blockDim.y + threadIdx.y;
     if (j >= g.ny) return;
     const int i = 2 * tt + ((j + colour) & 1);
     if (i >= g.nx) return;
@@ -627,6 +627,14 @@ __global__ __launch_bounds__(256) void k_sweep_typed_lf(Geo g, Typed<T> y, const
         sweep_column_typed_loaded<T, NZ, true>(g, y, x, c, rep, bv, xw, xe, xs, xn);
     else
         sweep_column_typed_loaded<T, NZ, false>(g, y, x, c, ty, bv, xw, xe, xs, xn);
+}
+template <class T, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_sweep_typed_lf(Geo g, Typed<T> y, const T* __restrict__ b,
+                                                        T* __restrict__ x, int colour, Geo gc,
+                                                        const T* __restrict__ xc, State* st,
+                                                        const double* __restrict__ cons_part, int cons_nblk,
+                                                        int cons_mode) {
+    sweep_typed_lf_body<T, NZ, MODE, kColsX>(hw_blk(), g, y, b, x, colour, gc, xc, st, cons_part, cons_nblk, cons_mode);
 }
 
 // restrict_sum_fast with the red children's faces from their types (each
@@ -717,12 +725,12 @@ __global__ __launch_bounds__(kPPThreads) void k_restrict_zero_pp(Geo f, Ops<T> f
 // 20.4 -> 13.7 us back to back, 33.9 -> 26.5 us with the caches flushed; the
 // same sums, so the same values.
 constexpr int kRestrictPlanes = 2;
-template <class T, int P>
+template <class T, int P, int BX = 0>
 __device__ __forceinline__ void restrict_fast_typed_body(const Blk& B, const Geo& f, const Typed<T>& y,
                                                          const T* __restrict__ fx, const Geo& cg, T* __restrict__ cb,
                                                          const State* __restrict__ st) {
     if (st->done) return;
-    const int I = B.x * blockDim.x + threadIdx.x;
+    const int I = B.x0<BX>() + threadIdx.x;
     const int J = B.y * blockDim.y + threadIdx.y;
     if (I >= cg.nx || J >= cg.ny) return;
     const bool two = 2 * I + 1 < f.nx && 2 * J + 1 < f.ny;
@@ -773,7 +781,7 @@ __device__ __forceinline__ void restrict_fast_typed_body(const Blk& B, const Geo
 template <class T>
 __global__ __launch_bounds__(256) void k_restrict_fast_typed(Geo f, Typed<T> y, const T* __restrict__ fx, Geo cg,
                                                              T* __restrict__ cb, const State* __restrict__ st) {
-    restrict_fast_typed_body<T, kRestrictPlanes>(hw_blk(), f, y, fx, cg, cb, st);
+    restrict_fast_typed_body<T, kRestrictPlanes, kColsX>(hw_blk(), f, y, fx, cg, cb, st);
 }
 
 // ---- coarse tail: the lowest levels of the V-cycle in one workgroup --------

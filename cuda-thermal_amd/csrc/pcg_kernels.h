@@ -180,7 +180,7 @@ __device__ __forceinline__ void residual_typed_body(const Blk& B, const Geo& g, 
             else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v, bw);
         }
     }
-    block_partials<3>(v, part, B.flat());
+    block_partials<3>(v, part, B.flat_xy());
 }
 template <class T, int NZ, int MODE>
 __global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
@@ -328,7 +328,7 @@ __device__ __forceinline__ void apply_typed_body(const Blk& B, const Geo& g, con
             v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true, false>(g, y, p, Ap, c, ty, inv_dt)
                                  : apply_column_typed<T, NZ, PT, false, false>(g, y, p, Ap, c, ty, inv_dt);
     }
-    block_partials<1>(v, part, B.flat());
+    block_partials<1>(v, part, B.flat_xy());
 }
 // 64 x 4 blocks. FP64 faces: the compiler's own register choice (106 us at
 // 1024^2 x 16 with the caches flushed, against 116 for 64 x 16 blocks).
