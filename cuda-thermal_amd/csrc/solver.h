@@ -135,6 +135,9 @@ struct Solver final : SolverBase {
     State* h_st = nullptr;  // pinned mirror
     std::vector<DevLevel<T>> L;
     hipGraphExec_t g_start = nullptr, g_iter = nullptr;
+    hipGraphExec_t g_start_P = nullptr;  // g_start reading P as the right-hand side
+    bool rhs_P = false;                  // set_rhs_steady: b is P, not copied (rhs())
+    const double* rhs() const { return rhs_P ? P : b; }
     // An iteration in two parts (split_iter: multigrid with a V-cycle whose
     // head is a 256-thread sweep): apply, update and the convergence test;
     // then V-cycle, dot and p update. mid_iter: the first part of the last
@@ -146,6 +149,7 @@ struct Solver final : SolverBase {
     ~Solver() override {
         if (s) (void)hipStreamSynchronize(s);
         if (g_start) (void)hipGraphExecDestroy(g_start);
+        if (g_start_P) (void)hipGraphExecDestroy(g_start_P);
         if (g_iter) (void)hipGraphExecDestroy(g_iter);
         if (g_iter_a) (void)hipGraphExecDestroy(g_iter_a);
         if (g_iter_b) (void)hipGraphExecDestroy(g_iter_b);
@@ -333,7 +337,9 @@ struct Solver final : SolverBase {
         lap(setup.factor_ms);
         split_iter = mg() && tail_k0 > 0;
         if (opt.use_graphs && mg()) {
-            if ((rc = capture(&g_start, true))) return rc;
+            if ((rc = capture(&g_start, true)) ||
+                (rc = capture_fn(&g_start_P, [&]() { return enqueue_start(false, true); })))
+                return rc;
             if ((rc = capture(&g_iter, false))) return rc;
             if (split_iter && ((rc = capture_fn(&g_iter_a, [&]() { return enqueue_iter_a(); })) ||
                                (rc = capture_fn(&g_iter_b, [&]() { return enqueue_iter_b(); }))))
@@ -463,16 +469,24 @@ struct Solver final : SolverBase {
         xz_device = false;
         return upload_split<double, double>(g0, g0.nz, host, x);
     }
+    // b = P (pcg3d_gpu.cu's copy) without the copy: until b is next written,
+    // the kernels that read the right-hand side read P itself (rhs()), the
+    // start from a graph of its own. set_power keeps the copy's meaning.
     int set_rhs_steady() override {
-        HIPTRY(hipMemcpyAsync(b, P, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        rhs_P = true;
         return TH_OK;
     }
     int set_rhs_transient() override {
+        rhs_P = false;
         k_rhs_transient<T><<<vec_blocks(n0), kVecThreads, 0, s>>>(n0, L[0].cap, P, x, b, st);
         HIPTRY(hipGetLastError());
         return TH_OK;
     }
     int set_power(const double* host) override {
+        if (rhs_P) {  // b keeps the power it was set from
+            HIPTRY(hipMemcpyAsync(b, P, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+            rhs_P = false;
+        }
         HIPTRY(hipStreamSynchronize(s));
         return upload_split<double, double>(g0, g0.nz, host, P);
     }
@@ -758,12 +772,14 @@ struct Solver final : SolverBase {
     // Start: residual, first preconditioner application, rz, p = z. rhs: a
     // backward-Euler step's b = P + C/dt x formed by the residual kernel
     // itself (multigrid).
-    int enqueue_start(bool rhs = false) {
+    // from_P: the right-hand side is P (set_rhs_steady).
+    int enqueue_start(bool rhs = false, bool from_P = false) {
         const dim3 grid = stencil_grid(2 * g0.hx, g0.ny);
         const unsigned vb = vec_blocks(n0);
+        const double* bb = from_P ? P : b;
         int rc;
         if (mg()) {
-            if ((rc = rhs ? launch_residual<RES_RHS>(P, x, part_res, b) : launch_residual<RES_MG>(b, x, part_res)))
+            if ((rc = rhs ? launch_residual<RES_RHS>(P, x, part_res, b) : launch_residual<RES_MG>(bb, x, part_res)))
                 return rc;
             if ((rc = vcycle(0, Cons{part_res, (int)nblk_res, CONS_START}))) return rc;
             k_dot<T><<<vb, kVecThreads, 0, s>>>(n0, r, L[0].x, st, part_dot);
@@ -772,7 +788,7 @@ struct Solver final : SolverBase {
             HIPTRY(hipGetLastError());
             return TH_OK;
         }
-        k_residual<T, RES_JACOBI><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), b, x, r, nullptr, invd, z, p, st,
+        k_residual<T, RES_JACOBI><<<grid, stencil_block(), 0, s>>>(g0, L[0].ops(), bb, x, r, nullptr, invd, z, p, st,
                                                                    part_res, nullptr);
         HIPTRY(hipGetLastError());
         return finalize<3, FIN_RES_JACOBI>(nblk_stencil, part_res);
@@ -844,11 +860,11 @@ struct Solver final : SolverBase {
 
     int run_start() {
         mid_iter = false;
-        if (g_start) {
-            HIPTRY(hipGraphLaunch(g_start, s));
+        if (hipGraphExec_t gs = rhs_P ? g_start_P : g_start) {
+            HIPTRY(hipGraphLaunch(gs, s));
             return TH_OK;
         }
-        return enqueue_start();
+        return enqueue_start(false, rhs_P);
     }
     int run_iter() {
         if (g_iter) {
@@ -891,7 +907,7 @@ struct Solver final : SolverBase {
     int replace_residual() {
         k_clear_xz<<<1, 1, 0, s>>>(st);  // x is no longer 0
         HIPTRY(hipGetLastError());
-        return launch_residual<RES_MG>(b, x, part_res);
+        return launch_residual<RES_MG>(rhs(), x, part_res);
     }
 
     // PCG iterations until the device reports done. Iterations are queued in
@@ -995,7 +1011,7 @@ struct Solver final : SolverBase {
             // kernel writes nothing (RES_TRUE): the start graph of a restart
             // computes r and the V-cycle's b itself. (With RES_MG's writes
             // the test took 135 us at 1024^2 x 16, without them 102.)
-            if ((rc = launch_residual<RES_TRUE>(b, x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
+            if ((rc = launch_residual<RES_TRUE>(rhs(), x, part_res)) || (rc = finalize<3, FIN_RES>(nblk_res, part_res)) ||
                 (rc = fetch_state()))
                 return rc;
             if (h_st->done) break;
@@ -1072,6 +1088,7 @@ struct Solver final : SolverBase {
         using clk = std::chrono::high_resolution_clock;
         int rc;
         hipGraphExec_t gh = nullptr, gt = nullptr;
+        rhs_P = false;  // every step's head writes b
         auto cleanup = [&]() {
             if (gh) (void)hipGraphExecDestroy(gh);
             if (gt) (void)hipGraphExecDestroy(gt);

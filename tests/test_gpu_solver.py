@@ -285,6 +285,32 @@ def test_set_power_and_warm_start():
     assert np.abs(x2 - 2.0 * x1).max() < 1e-8 * x2.max()
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_rhs_steady_is_a_copy_of_p(graphs):
+    """set_rhs_steady sets b = P (the solve reads P in place until b is next
+    written): set_power after it leaves b at the old power, as the copy
+    would, and the next set_rhs_steady takes the new one. The true residual
+    check and its restart read the same right-hand side."""
+    m = hotspot(48)
+    P = m.get("P")
+    ref = T.Solver(m, graphs=graphs, true_residual=True)
+    ref.set_x_zero()
+    ref.set_rhs_steady()
+    st0 = ref.solve(1e-10)
+    x0 = ref.get_x()
+    s = T.Solver(m, graphs=graphs, true_residual=True)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    s.set_power(3.0 * P)
+    st1 = s.solve(1e-10)
+    assert st1["iters"] == st0["iters"] and np.array_equal(s.get_x(), x0)
+    s.set_x_zero()
+    s.set_rhs_steady()
+    st2 = s.solve(1e-10)
+    assert st2["converged"] and s.true_relres() < 1e-10
+    assert np.abs(s.get_x() - 3.0 * x0).max() < 1e-8 * 3.0 * x0.max()
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_true_residual_restart(precision):
     """With the true-residual check, the solve only reports convergence once
