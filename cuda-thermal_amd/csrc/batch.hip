@@ -133,10 +133,11 @@ __global__ __launch_bounds__(kTailLdsThreads) void kb_tail(const Inst* __restric
                              reinterpret_cast<float*>(smem));
 }
 
-// Level-0 stencils. Instances whose level 0 has column types run the typed
-// bodies in blocks of 64 x 4 (as the single-instance solver does: the same
-// partial sums) from a table of their own; the others the per-cell bodies
-// in blocks of 64 x 16.
+// Level-0 stencils in blocks of 64 x 16. Instances whose level 0 has column
+// types run the typed bodies from a table of their own, each block writing
+// the partials of the four 64 x 4 blocks the single-instance solver runs in
+// its place (the same sums); the others the per-cell bodies.
+static_assert(kStencilRows % kColsY == 0 && kStencilRows <= 16, "whole row groups, at most 16 waves per block");
 __global__ __launch_bounds__(1024) void kb_residual(const Inst* __restrict__ I, const Entry* __restrict__ tab,
                                                     int ntab) {
     Blk B;
@@ -146,13 +147,13 @@ __global__ __launch_bounds__(1024) void kb_residual(const Inst* __restrict__ I, 
     residual_body<float, RES_MG>(B, g, o, in.b, in.x, in.r, in.L[0].b, nullptr, nullptr, nullptr, in.st, in.part_res);
 }
 template <int NZ>
-__global__ __launch_bounds__(256) void kb_residual_typed(const Inst* __restrict__ I, const Entry* __restrict__ tab,
-                                                         int ntab) {
+__global__ __launch_bounds__(1024) void kb_residual_typed(const Inst* __restrict__ I, const Entry* __restrict__ tab,
+                                                          int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
     const Geo g = in.g0;  // copies, as in kb_sweep
     const Typed<float> y = in.L[0].y;
-    residual_typed_body<float, NZ, RES_MG>(B, g, y, in.b, in.x, in.r, in.L[0].b, in.st, in.part_res, nullptr);
+    residual_typed_body<float, NZ, RES_MG, true>(B, g, y, in.b, in.x, in.r, in.L[0].b, in.st, in.part_res, nullptr);
 }
 
 template <int NZ>
@@ -163,15 +164,14 @@ __global__ __launch_bounds__(1024) void kb_apply(const Inst* __restrict__ I, con
     const Ops<float> o = in.L[0].o;
     apply_body<float, NZ, float>(B, g, o, in.p, in.Ap, in.st, in.part_apply);
 }
-// at least 6 waves per SIMD, as k_apply_typed_occ
 template <int NZ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_apply_typed(
-    const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
+__global__ __launch_bounds__(1024) void kb_apply_typed(const Inst* __restrict__ I, const Entry* __restrict__ tab,
+                                                       int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
     const Geo g = in.g0;  // copies, as in kb_sweep
     const Typed<float> y = in.L[0].y;
-    apply_typed_body<float, NZ, float>(B, g, y, in.p, in.Ap, in.st, in.part_apply);
+    apply_typed_body<float, NZ, float, true>(B, g, y, in.p, in.Ap, in.st, in.part_apply);
 }
 
 __global__ __launch_bounds__(1024) void kb_update(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
@@ -403,10 +403,11 @@ struct Batch {
         for (int i = 0; i < n; ++i) {
             Inst& d = hInst[i];
             const int sb = (int)((2L * d.g0.hx + kColsX - 1) / kColsX), sr = (d.g0.ny + kStencilRows - 1) / kStencilRows;
-            const int tr = d.L[0].ntypes ? (d.g0.ny + kColsY - 1) / kColsY : sr;
-            if (sb * tr != d.nblk_stencil)  // the consumers read this many partials
+            // the consumers read this many partials: of 64 x 4 blocks with types
+            const int pr = d.L[0].ntypes ? (d.g0.ny + kColsY - 1) / kColsY : sr;
+            if (sb * pr != d.nblk_stencil)
                 return fail(TH_ERR_ARG, "batched solver: level-0 blocks differ from the instance's partials");
-            add(d.L[0].ntypes ? t_typed : t_stencil, i, sb, tr, 1);
+            add(d.L[0].ntypes ? t_typed : t_stencil, i, sb, sr, 1);
             const int vb = (int)std::max(1L, std::min((long)kMaxVecBlocks, (d.n0 + vt - 1) / vt));
             add(t_vec, i, vb, 1, 1);
             d.nblk_vec = vb;
@@ -499,7 +500,7 @@ struct Batch {
         if (t_stencil.blocks)
             kb_residual<<<t_stencil.blocks, dim3(kColsX, kStencilRows), 0, s>>>(dI, t_stencil.d, (int)t_stencil.h.size());
         if (t_typed.blocks) {
-#define TB_RS(NZ) kb_residual_typed<NZ><<<t_typed.blocks, dim3(kColsX, kColsY), 0, s>>>(dI, t_typed.d, (int)t_typed.h.size())
+#define TB_RS(NZ) kb_residual_typed<NZ><<<t_typed.blocks, dim3(kColsX, kStencilRows), 0, s>>>(dI, t_typed.d, (int)t_typed.h.size())
             switch (bucket(nz)) {
                 case 4: TB_RS(4); break;
                 case 8: TB_RS(8); break;
@@ -523,7 +524,7 @@ struct Batch {
         default: TB_AP(K, TB, ROWS, 16); break; \
     }
         if (t_stencil.blocks) TB_APZ(kb_apply, t_stencil, kStencilRows);
-        if (t_typed.blocks) TB_APZ(kb_apply_typed, t_typed, kColsY);
+        if (t_typed.blocks) TB_APZ(kb_apply_typed, t_typed, kStencilRows);
 #undef TB_APZ
 #undef TB_AP
         HIPTRY(hipGetLastError());

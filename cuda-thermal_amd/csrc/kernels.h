@@ -260,6 +260,30 @@ __device__ void block_partials(double (&v)[NV], double* __restrict__ part, int b
         part[(size_t)bid * NV + tid] = s;
     }
 }
+// The partials of blocks of 64 x kColsY threads from a block of 64 x
+// (m kColsY) threads (one wave per row) that runs in place of m of them: its
+// rows in m groups of kColsY waves, each group summed as block_partials sums
+// such a block, into the slot that block has (x + gx * row block), for the
+// row blocks below nrb.
+template <int NV>
+__device__ void row_group_partials(double (&v)[NV], double* __restrict__ part, const Blk& B, int nrb) {
+    __shared__ double sh[NV][16];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    const int m = blockDim.y / kColsY;
+    const int q = tid / NV, k = tid - q * NV, rb = B.y * m + q;
+    if (q < m && rb < nrb) {
+        double s = sh[k][q * kColsY];
+        for (int i = 1; i < kColsY; ++i) s += sh[k][q * kColsY + i];
+        part[((size_t)B.x + (size_t)B.gx * rb) * NV + k] = s;
+    }
+}
 template <int NV, unsigned KMAX = 0>
 __device__ void block_partials(double (&v)[NV], double* __restrict__ part) {
     block_partials<NV, KMAX>(v, part, (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));

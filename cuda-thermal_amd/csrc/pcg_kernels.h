@@ -153,7 +153,9 @@ __device__ __forceinline__ void residual_column_typed(const Geo& g, const Typed<
         gdn = Gu;
     }
 }
-template <class T, int NZ, int MODE>
+// ROWS: blocks of 64 x (m kColsY) threads whose partials are those of m
+// blocks of 64 x kColsY (row_group_partials).
+template <class T, int NZ, int MODE, bool ROWS = false>
 __device__ __forceinline__ void residual_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
                                                     const double* __restrict__ b, const double* __restrict__ x,
                                                     double* __restrict__ r, T* __restrict__ b0,
@@ -180,7 +182,8 @@ __device__ __forceinline__ void residual_typed_body(const Blk& B, const Geo& g, 
             else residual_column_typed<T, NZ, MODE, false, false, false>(g, y, b, x, r, b0, c, ty, inv_dt, v, bw);
         }
     }
-    block_partials<3>(v, part, B.flat_xy());
+    if constexpr (ROWS) row_group_partials<3>(v, part, B, (g.ny + kColsY - 1) / kColsY);
+    else block_partials<3>(v, part, B.flat_xy());
 }
 template <class T, int NZ, int MODE>
 __global__ __launch_bounds__(256) void k_residual_typed(Geo g, Typed<T> y, const double* __restrict__ b,
@@ -307,7 +310,7 @@ __device__ __forceinline__ double apply_column_typed(const Geo& g, const Typed<T
     }
     return v;
 }
-template <class T, int NZ, class PT>
+template <class T, int NZ, class PT, bool ROWS = false>  // ROWS: as residual_typed_body
 __device__ __forceinline__ void apply_typed_body(const Blk& B, const Geo& g, const Typed<T>& y,
                                                  const PT* __restrict__ p, double* __restrict__ Ap,
                                                  const State* __restrict__ st, double* __restrict__ part) {
@@ -328,7 +331,8 @@ __device__ __forceinline__ void apply_typed_body(const Blk& B, const Geo& g, con
             v[0] = inv_dt != 0.0 ? apply_column_typed<T, NZ, PT, true, false>(g, y, p, Ap, c, ty, inv_dt)
                                  : apply_column_typed<T, NZ, PT, false, false>(g, y, p, Ap, c, ty, inv_dt);
     }
-    block_partials<1>(v, part, B.flat_xy());
+    if constexpr (ROWS) row_group_partials<1>(v, part, B, (g.ny + kColsY - 1) / kColsY);
+    else block_partials<1>(v, part, B.flat_xy());
 }
 // 64 x 4 blocks. FP64 faces: the compiler's own register choice (106 us at
 // 1024^2 x 16 with the caches flushed, against 116 for 64 x 16 blocks).
