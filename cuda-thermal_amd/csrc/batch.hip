@@ -54,16 +54,23 @@ struct Entry {
     int inst, begin, gx, gy, gz;
 };
 
-// Instance and block coordinates of this block in a stage's table (binary
-// search on the first blocks; the table covers the launch exactly).
+// Instance and block coordinates of this block in a stage's table (the
+// table covers the launch exactly, its entries in block order): the last
+// entry whose first block is <= this one. Each wave tests 64 entries with one
+// load per lane and a ballot (the entries that pass are a prefix), so a
+// block start waits for one load per 64 instances instead of a binary
+// search's chain of dependent loads. Every lane of the wave must be active.
 __device__ __forceinline__ const Inst& locate(const Inst* __restrict__ I, const Entry* __restrict__ tab, int n,
                                               Blk& B) {
     const int b = blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[mid].begin <= b) lo = mid;
-        else hi = mid - 1;
+    const int lane = (int)((threadIdx.y * blockDim.x + threadIdx.x) & 63);
+    int lo = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int k = base + lane;
+        const unsigned long long m = __ballot(k < n && tab[k].begin <= b);
+        const int c = __popcll(m);
+        if (c) lo = base + c - 1;
+        if (c < 64) break;
     }
     const Entry e = tab[lo];
     int r = b - e.begin;
