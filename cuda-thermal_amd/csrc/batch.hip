@@ -122,8 +122,8 @@ __global__ __launch_bounds__(512) void kb_restrict(const Inst* __restrict__ I, c
     const Geo gc = in.L[k + 1].g;
     float* bc = in.L[k + 1].b;
     const State* st = in.st;
-    if (L.ntypes)  // one plane per block (the table's z), as restrict_fast_body
-        restrict_fast_typed_body<float, 1>(B, L.g, L.y, L.x, gc, bc, st);
+    if (L.ntypes)  // kRestrictPlanes planes per block z (the table's z), as k_restrict_fast_typed
+        restrict_fast_typed_body<float, kRestrictPlanes>(B, L.g, L.y, L.x, gc, bc, st);
     else
         restrict_fast_body<float>(B, L.g, L.o, L.x, gc, bc, st);
 }
@@ -422,7 +422,8 @@ struct Batch {
                 const Geo& g = d.L[k].g;
                 const Geo& c = d.L[k + 1].g;
                 add(t_sweep[k], i, (g.hx + cw - 1) / cw, (g.ny + br - 1) / br, 1);
-                add(t_restrict[k], i, (c.nx + cw - 1) / cw, (c.ny + br - 1) / br, c.nz);
+                add(t_restrict[k], i, (c.nx + cw - 1) / cw, (c.ny + br - 1) / br,
+                    d.L[k].ntypes ? (c.nz + kRestrictPlanes - 1) / kRestrictPlanes : c.nz);
             }
         }
         HIPTRY(hipMemcpy(dI, hInst.data(), n * sizeof(Inst), hipMemcpyHostToDevice));
