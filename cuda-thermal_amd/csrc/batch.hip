@@ -41,7 +41,7 @@ struct Inst {
     Geo g0;
     long n0;
     const double* P;
-    double *x, *b, *r, *Ap;
+    double *x, *r, *Ap;
     float* p;  // search direction (FP32, as the single-instance mixed solver)
     double *part_res, *part_apply, *part_upd, *part_dot;
     int nblk_stencil, nblk_vec;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(1024) void kb_residual(const Inst* __restrict__ I, 
     const Inst& in = locate(I, tab, ntab, B);
     const Geo g = in.g0;  // copies, as in kb_sweep
     const Ops<float> o = in.L[0].o;
-    residual_body<float, RES_MG>(B, g, o, in.b, in.x, in.r, in.L[0].b, nullptr, nullptr, nullptr, in.st, in.part_res);
+    residual_body<float, RES_MG>(B, g, o, in.P, in.x, in.r, in.L[0].b, nullptr, nullptr, nullptr, in.st, in.part_res);
 }
 template <int NZ>
 __global__ __launch_bounds__(1024) void kb_residual_typed(const Inst* __restrict__ I, const Entry* __restrict__ tab,
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(1024) void kb_residual_typed(const Inst* __restrict
     const Inst& in = locate(I, tab, ntab, B);
     const Geo g = in.g0;  // copies, as in kb_sweep
     const Typed<float> y = in.L[0].y;
-    residual_typed_body<float, NZ, RES_MG, true>(B, g, y, in.b, in.x, in.r, in.L[0].b, in.st, in.part_res, nullptr);
+    residual_typed_body<float, NZ, RES_MG, true>(B, g, y, in.P, in.x, in.r, in.L[0].b, in.st, in.part_res, nullptr);
 }
 
 template <int NZ>
@@ -203,13 +203,12 @@ __global__ __launch_bounds__(1024) void kb_pupdate(const Inst* __restrict__ I, c
                                       in.nblk_vec);
 }
 
-// Steady right-hand side and zero start for every instance: b = P, x = 0.
+// Zero start for every instance: x = 0 (the start residual reads P as b).
 __global__ __launch_bounds__(1024) void kb_reset(const Inst* __restrict__ I, const Entry* __restrict__ tab, int ntab) {
     Blk B;
     const Inst& in = locate(I, tab, ntab, B);
     const long stride = (long)B.gx * blockDim.x;
     for (long k = (long)B.x * blockDim.x + threadIdx.x; k < in.n0; k += stride) {
-        in.b[k] = in.P[k];
         in.x[k] = 0.0;
     }
 }
@@ -228,10 +227,12 @@ __global__ __launch_bounds__(1024) void kb_finish(const Inst* __restrict__ I, co
     }
 }
 
-__global__ void kb_begin(const Inst* __restrict__ I, int n, double tol, int maxit) {
+// xz: the start residual is b = P itself (x = 0, reset before the start).
+__global__ void kb_begin(const Inst* __restrict__ I, int n, double tol, int maxit, int xz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     State* st = I[i].st;
+    st->xz = xz;
     st->xlag = 0;
     st->tol = tol;
     st->maxit = maxit;
@@ -341,7 +342,6 @@ struct Batch {
             d.n0 = x.n0;
             d.P = x.P;
             d.x = x.x;
-            d.b = x.b;
             d.r = x.r;
             d.p = x.pm;
             d.Ap = x.Ap;
@@ -555,7 +555,7 @@ struct Batch {
         using clk = std::chrono::high_resolution_clock;
         HIPTRY(hipStreamSynchronize(s));
         const auto t0 = clk::now();
-        kb_begin<<<(n + 63) / 64, 64, 0, s>>>(dI, n, tol, maxit);
+        kb_begin<<<(n + 63) / 64, 64, 0, s>>>(dI, n, tol, maxit, 1);
         HIPTRY(hipGetLastError());
         int rc;
         if ((rc = vec(kb_reset))) return rc;
@@ -608,7 +608,7 @@ struct Batch {
         HIPTRY(hipEventCreate(&e0));
         HIPTRY(hipEventCreate(&e1));
         auto timed = [&](auto&& fn, double* out) -> int {
-            kb_begin<<<(n + 63) / 64, 64, 0, s>>>(dI, n, 0.0, 1 << 30);
+            kb_begin<<<(n + 63) / 64, 64, 0, s>>>(dI, n, 0.0, 1 << 30, 0);
             HIPTRY(hipGetLastError());
             int rc = fn();
             if (rc) return rc;
